@@ -1,0 +1,178 @@
+"""Throughput of the multi-colour sweep on one MI355X: site updates/s and fraction of the HBM roofline.
+
+Default workload (the configuration the headline metric is quoted on): bitwise multi-spin Ising, simple cubic
+L = 128 periodic, J = 1, h = 0, B = 1024 replicas packed 32 per uint32 word with independent per-replica Philox
+draws, T = 4.5, random +-1 start. A step is one coloured sweep (2 class launches + the counter bump) over all
+replicas; inputs are resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--workload bitwise|ising|heisenberg|xy]
+
+Prints one JSON line. `roofline` is for the dominant kernel (the class-update kernel): algorithmic bytes per
+launch (compulsory read + write of each spin word: 0.25 B per site update) over the launch's average duration,
+measured with HIP events on the stream the kernels run on. `cpu_baseline` times the CPU oracle (the same chain,
+sequential, in C) on a bounded sample on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+for p in (str(REPO), str(REPO / "spin-colour-kernel_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (description, builder, sampler kwargs, B, temperatures, algorithmic bytes per update)
+    "bitwise": "bitwise Ising sc L=128 periodic, B=1024 (32 words), independent draws, T=4.5",
+    "ising": "Ising square L=64 periodic, int8, B=512 (8 temps x 64 in [1.5,3.5]), seed 0",
+    "heisenberg": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, fp32, B=8192 (32 temps x 256 in [0.5,3.0])",
+    "xy": "XY triangular L=2048 periodic J1=1, fp32, B=128 replica block (16 temps x 8 in [0.7,1.1])",
+}
+
+
+def _make(workload: str):
+    import numpy as np
+    import torch
+
+    from sck_amd import crystals as cr
+    from sck_amd import system as sy
+    if workload == "bitwise":
+        s = sy.ising("sc", 128)
+        return s, dict(bitwise=True, independent=True), 1024, torch.full((1024,), 4.5), 0.25
+    if workload == "ising":
+        s = sy.ising("square", 64)
+        T = torch.as_tensor(np.repeat(np.linspace(1.5, 3.5, 8).astype(np.float32), 64))
+        return s, dict(packed=False), 512, T, 2.0
+    if workload == "heisenberg":
+        s = sy.System("Heisenberg bcc J1J2", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
+                      {1: 1.0, 2: -0.2}, h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0)))
+        T = torch.as_tensor(np.repeat(np.linspace(0.5, 3.0, 32).astype(np.float32), 256))
+        return s, {}, 8192, T, 24.0
+    if workload == "xy":
+        s = sy.xy("triangular", 2048)
+        T = torch.as_tensor(np.repeat(np.linspace(0.7, 1.1, 16).astype(np.float32), 8))
+        return s, {}, 128, T, 16.0
+    raise SystemExit(f"unknown workload {workload!r}; known: {', '.join(WORKLOADS)}")
+
+
+def prebuilt_specs():
+    """(system, mode, batch) of the default workload, for the build step's kernel pre-compilation."""
+    from sck_amd import hipgen as G
+    s, kw, B, _, _ = _make("bitwise")
+    return [(s, G.MODE_BITWISE_INDEP, B)]
+
+
+def cpu_baseline(workload: str, s, A, S0, budget_s: float = 12.0) -> dict:
+    """Time the CPU oracle (sequential chain in C, one thread) on a bounded sample of the same workload."""
+    import numpy as np
+
+    import oracle as O
+    os_ = O.OracleSystem(s)
+    if A.bitwise:
+        x = O.unpack(S0[0:1], 32, s.site_mask())[:, 0]
+        tab = A.ptab.cpu().numpy()[0:1]
+        t0 = time.perf_counter()
+        sweeps = 0
+        while time.perf_counter() - t0 < budget_s and sweeps < 50:
+            x = O.bitwise_chain(os_, x, tab, 0, sweeps, 1, independent=A.independent, threads=1)
+            sweeps += 1
+        dt = time.perf_counter() - t0
+        return {"value": 32 * s.n_active * sweeps / dt, "unit": "site updates/s", "cores": 1, "kind": "port",
+                "sample": f"1 word (32 replicas) x {s.n_active} sites x {sweeps} sweeps, sequential chain, "
+                          f"{dt:.1f} s"}
+    reps = [0]
+    beta = A.beta.cpu().numpy()
+    x = S0[reps].astype(np.float32)
+    t0 = time.perf_counter()
+    sweeps = 0
+    while time.perf_counter() - t0 < budget_s and sweeps < 1000:
+        x, _ = O.chain(os_, x, beta, 0, sweeps, 1, reps=reps)
+        sweeps += 1
+    dt = time.perf_counter() - t0
+    return {"value": len(reps) * s.n_active * sweeps / dt, "unit": "site updates/s", "cores": 1, "kind": "port",
+            "sample": f"{len(reps)} replica x {s.n_active} sites x {sweeps} sweeps, sequential chain, {dt:.1f} s"}
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch of the class kernel from the committed rocprofv3 counter summary, if any."""
+    f = REPO / "profiles" / f"traffic_{workload}.json"
+    if f.exists():
+        return json.loads(f.read_text()).get("bytes_per_launch")
+    return None
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="bitwise", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build measures one GPU: the sweep path has no multi-GPU component "
+                          "(replica blocks shard across GPUs without communication); run with --gpus 1"}))
+        sys.exit(2)
+    import torch
+
+    from sck_amd.sampler import Sampler
+    s, kw, B, T, alg_bytes = _make(a.workload)
+    A = Sampler(s, B, "cuda:0", seed=0, **kw)
+    A.set_temperature(T)
+    S = A.init_spins(seed=0)
+    S0 = S.cpu().numpy()
+    ncls = len(A.classes)
+    upd_per_sweep = B * s.n_active
+    A.sweep(S, a.warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    A.sweep(S, a.steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    sec = ms / 1e3
+    value = upd_per_sweep * a.steps / sec
+    # dominant kernel: the class launches; the counter bump (one thread per sweep) is negligible
+    launch_s = sec / (a.steps * ncls)
+    bytes_per_launch = upd_per_sweep / ncls * alg_bytes
+    achieved = bytes_per_launch / launch_s / 1e9
+    out = {
+        "metric": "site updates/s (%HBM roofline)",
+        "value": value,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bitwise": "u32", "ising": "int8", "heisenberg": "f32", "xy": "f32"}[a.workload],
+        "data": "synthetic (random initial spins from the seeded torch CPU generator)",
+        "config": {"workload": WORKLOADS[a.workload], "batch": B, "sites": s.n_active, "classes": ncls,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload),
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6},
+        "wall_s": wall,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.workload, s, A, S0)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

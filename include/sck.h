@@ -1,0 +1,134 @@
+/* sck.h -- C-ABI of libsck.so, the MI355X backend of the multi-colour Metropolis sweep.
+ *
+ * The reference has no native boundary: its kernels are Triton source generated per system
+ * (SCK/generator.py:322-337 compile_system, SCK/bitwise.py:250-335) and launched by the Sampler operator class
+ * (SCK/sampler.py:39-345). This header is the boundary that replaces that pair: the host side generates HIP
+ * source per system, and libsck compiles it with hipRTC for gfx950, launches its class kernels, captures
+ * sweeps in HIP graphs and runs the observable reductions. Plain pointers and sizes only; every device buffer
+ * (spins, mask, k1, beta, tables, state) belongs to the caller and is read by address, so buffers updated in
+ * place are seen by graphs captured earlier (SCK/sampler.py:134-150).
+ *
+ * Errors: every entry point returns 0 on success and a negative SCK_E* code otherwise; sck_last_error()
+ * returns the calling thread's message for the last failure.
+ */
+#ifndef SCK_H
+#define SCK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCK_ABI_VERSION 1
+
+enum sck_status {
+    SCK_OK = 0,
+    SCK_E_INVALID = -1,   /* bad argument (ValueError on the Python side) */
+    SCK_E_COMPILE = -2,   /* hipRTC compilation failed (log in sck_last_error) */
+    SCK_E_HIP = -3,       /* a HIP runtime call failed */
+    SCK_E_NOMEM = -4
+};
+
+/* Spin storage / kernel family of a module (SCK/sampler.py:40-69). */
+enum sck_mode {
+    SCK_MODE_INT8 = 0,          /* Ising, int8 +-1 per replica               */
+    SCK_MODE_F32 = 1,           /* float32 components (Ising stored as float, XY, Heisenberg) */
+    SCK_MODE_PACKED = 2,        /* Ising, 32 replicas per int32 word, per-replica Philox draws */
+    SCK_MODE_BITWISE = 3,       /* bitwise multi-spin Ising, one uniform per (word, site) */
+    SCK_MODE_BITWISE_INDEP = 4  /* bitwise multi-spin Ising, one 32-bit U per replica */
+};
+
+/* Arguments every generated kernel receives, by value (one struct parameter).
+ * Mirrors the runtime arguments of the reference kernel kernel(src_ptr, dst_ptr, mask_ptr, k1_ptr, beta_ptr,
+ * state_ptr, x_only, row0, ...) (SCK/generator.py:191-194; SCK/bitwise.py:278-281). */
+typedef struct sck_args {
+    void *spins;              /* S [B][n][b][L3][L2][L1] (or words [W][1][b][L3][L2][L1]); updated in place */
+    void *dst;                /* synchronous control only: destination of the update            */
+    const int8_t *mask;       /* [b][L3][L2][L1], 1 = spin present                               */
+    const float *k1;          /* [b][L3][L2][L1] cubic K1 per site                               */
+    const float *beta;        /* [Bp] inverse temperature per replica                            */
+    const void *table;        /* acceptance table: Ising fp32 [Bp][T]; bitwise fp32/int64 [W][2][Z+1][Z+1] */
+    const int64_t *state;     /* [2] = (base seed, sweep counter)                                */
+    const float *width;       /* cone proposal: half-angle per replica [Bp], or the learned width table */
+    const float *site_ku;     /* [b][L3][L2][L1] per-site uniaxial strength                      */
+    const float *site_axis;   /* [b][L3][L2][L1][n] per-site uniaxial axis                        */
+    double *obs;              /* observables: output E [B] then M [B][b][n]; partial sums in obs_scratch */
+    double *obs_scratch;      /* [B][chunks][1 + b*n] per-block partial sums                      */
+    int64_t sweep_offset;     /* sweep index = state[1] + sweep_offset                            */
+    int32_t nrep;             /* replicas B (words W for packed/bitwise)                          */
+    int32_t flags;            /* reserved, 0                                                      */
+} sck_args;
+
+/* One kernel launch of a plan. */
+typedef struct sck_launch {
+    int32_t kernel;           /* index into the module's kernel list */
+    uint32_t grid[3];
+    uint32_t block[3];
+    uint32_t shmem;
+} sck_launch;
+
+/* Launch plans of a system, computed by the host-side generator. */
+typedef struct sck_desc {
+    int32_t abi_version;      /* SCK_ABI_VERSION */
+    int32_t mode;             /* enum sck_mode */
+    int32_t n, b, L1, L2, L3; /* model and box */
+    int32_t batch;            /* B, replicas */
+    int32_t words;            /* W = ceil(B/32) for packed/bitwise, else B */
+    int32_t obs_chunks;       /* observable partial sums per replica */
+    const char *const *kernel_names; int32_t n_kernels;
+    const sck_launch *sweep;  int32_t n_sweep;   /* one coloured sweep: class launches in sweep order */
+    const sck_launch *ref;    int32_t n_ref;     /* one sequential sweep (reference chain)           */
+    const sck_launch *sync;   int32_t n_sync;    /* one synchronous-control sweep                    */
+    const sck_launch *obs;    int32_t n_obs;     /* energy and magnetization                          */
+    const sck_launch *bump;                      /* state[1] += sweep_offset                          */
+} sck_desc;
+
+typedef struct sck_handle sck_handle;
+typedef struct sck_graph sck_graph;
+
+/* Library version and the device architecture the module cache is keyed on ("gfx950"). */
+int sck_version(void);
+const char *sck_last_error(void);
+
+/* Compile `hip_src` with hipRTC for gfx950 (or load it from the code-object cache `cache_dir`, NULL = no
+ * cache), load the module on the current device and resolve the kernels named in desc.
+ * Replaces compile_system (SCK/generator.py:322-337) plus Sampler.__init__'s launch geometry
+ * (SCK/sampler.py:70-131). */
+int sck_create(const sck_desc *desc, const char *hip_src, const char *cache_dir, sck_handle **out);
+
+/* Compile only (no device needed): writes the code object into cache_dir. Used to pre-build kernels. */
+int sck_precompile(const char *hip_src, const char *cache_dir);
+
+int sck_destroy(sck_handle *h);
+
+/* n coloured sweeps in place, then state[1] += n (SCK/sampler.py:234-240). stream: a hipStream_t or NULL.
+ * class_lo/class_hi select launches [lo, hi) of one sweep (0, -1: all); used by per-class checks. */
+int sck_sweep(sck_handle *h, const sck_args *args, int32_t n, int32_t class_lo, int32_t class_hi, void *stream);
+
+/* n sequential sweeps (one thread per replica or word visiting the sites in class order), then
+ * state[1] += n (SCK/sampler.py:281-293). */
+int sck_reference_sweep(sck_handle *h, const sck_args *args, int32_t n, void *stream);
+
+/* n synchronous-control sweeps: every active site from the old state (SCK/sampler.py:271-279).
+ * args->dst must hold a copy of args->spins' size; the result is copied back into args->spins. */
+int sck_sync_sweep(sck_handle *h, const sck_args *args, int32_t n, int64_t spin_bytes, void *stream);
+
+/* Float64 energy [B] and per-sublattice magnetization [B][b][n] into args->obs (SCK/sampler.py:326-345). */
+int sck_observe(sck_handle *h, const sck_args *args, void *stream);
+
+/* Capture k sweeps (launches and counter bump; with observe != 0 an observation after them) in a HIP graph
+ * (SCK/sampler.py:242-269). The graph reads the buffers of `args` by address. */
+int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t observe, sck_graph **out);
+int sck_graph_replay(sck_graph *g, void *stream);
+int sck_graph_destroy(sck_graph *g);
+
+/* Elapsed milliseconds of `reps` replays of a graph (or of k sweeps when g is NULL), timed with HIP events
+ * on `stream`. */
+int sck_time_replays(sck_graph *g, int32_t reps, void *stream, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -474,7 +474,8 @@ void oracle_sync(const oracle_sys *s, float *spins, const int64_t *reps, int64_t
  *   U < ptab_i[w][s][ze][k] as integers (SCK/bitwise.py:159-184, 236-245).
  * Z1 = Z + 1 is the table's inner size; field selects the s plane (0 otherwise). J-carrying bonds only. */
 void oracle_bitwise(const oracle_sys *s, int8_t *S, int64_t B, const float *ptab_f, const int64_t *ptab_i, int Z1,
-                    int field, int independent, int64_t base_seed, int64_t sweep0, int64_t nsweeps, int nthreads) {
+                    int field, int independent, int64_t base_seed, int64_t sweep0, int64_t nsweeps, int nthreads,
+                    int64_t word0) {
     int64_t nsite = nsite_of(s);
     int64_t pl = (int64_t)s->L[2] * s->L[1] * s->L[0];
     int nc = n_colours(s);
@@ -506,15 +507,15 @@ void oracle_bitwise(const oracle_sys *s, int8_t *S, int64_t B, const float *ptab
                                     for (int g = 0; g < 8; ++g) {
                                         int64_t kg = (seed + (int64_t)(g + 1) * 104729LL) % 2147483629LL;
                                         uint32_t rw[4];
-                                        philox_raw4(kg, (uint64_t)(uint32_t)(w * nsite + own), rw, 10);
+                                        philox_raw4(kg, (uint64_t)(uint32_t)((word0 + w) * nsite + own), rw, 10);
                                         for (int q = 0; q < 4; ++q)
                                             for (int j = 0; j < 32; ++j)
                                                 U[j] |= ((rw[q] >> j) & 1u) << (31 - (4 * g + q));
                                     }
                                 } else {
                                     uint32_t r4[4];
-                                    philox_raw4(seed, (uint64_t)(uint32_t)((w / 4) * nsite + own), r4, 10);
-                                    uf = philox_u01(r4[w % 4]);
+                                    philox_raw4(seed, (uint64_t)(uint32_t)(((word0 + w) / 4) * nsite + own), r4, 10);
+                                    uf = philox_u01(r4[(word0 + w) % 4]);
                                 }
                                 for (int j = 0; j < 32; ++j) {
                                     int64_t r = 32 * w + j;

@@ -126,7 +126,8 @@ class OracleSystem:
 
 def numpy_exp32(x: float) -> float:
     """float32 exp as the reference's CPU chain evaluates tl.exp (numpy on float32)."""
-    return float(np.exp(np.asarray([x], np.float32))[0])
+    with np.errstate(over="ignore"):
+        return float(np.exp(np.asarray([x], np.float32))[0])
 
 
 _EXP_CB = C.CFUNCTYPE(C.c_float, C.c_float)(numpy_exp32)
@@ -229,8 +230,9 @@ def p_fixed(s, beta) -> np.ndarray:
 
 
 def bitwise_chain(osys: OracleSystem, spins: np.ndarray, ptab: np.ndarray, base_seed: int, sweep0: int,
-                  nsweeps: int, independent: bool = False, threads: int = 1) -> np.ndarray:
-    """Sequential bitwise chain on unpacked spins (B, b, L3, L2, L1) int8 (SCK/bitwise.py:194-246)."""
+                  nsweeps: int, independent: bool = False, threads: int = 1, word_offset: int = 0) -> np.ndarray:
+    """Sequential bitwise chain on unpacked spins (B, b, L3, L2, L1) int8 (SCK/bitwise.py:194-246). The spins and
+    table may cover words word_offset .. word_offset + ceil(B/32) - 1 of a larger batch."""
     s = osys.s
     J, h, Z = _bitwise_params(s)
     S = np.ascontiguousarray(np.asarray(spins, np.int8).reshape(spins.shape[0], -1)).copy()
@@ -243,7 +245,8 @@ def bitwise_chain(osys: OracleSystem, spins: np.ndarray, ptab: np.ndarray, base_
         pi = np.zeros(1, np.int64)
     lib().oracle_bitwise(C.byref(osys.st), C.c_void_p(_ptr(S)), C.c_int64(B), C.c_void_p(_ptr(pf)),
                          C.c_void_p(_ptr(pi)), C.c_int(Z + 1), C.c_int(int(h != 0.0)), C.c_int(int(independent)),
-                         C.c_int64(int(base_seed)), C.c_int64(int(sweep0)), C.c_int64(int(nsweeps)), C.c_int(threads))
+                         C.c_int64(int(base_seed)), C.c_int64(int(sweep0)), C.c_int64(int(nsweeps)), C.c_int(threads),
+                         C.c_int64(int(word_offset)))
     return S.reshape(spins.shape)
 
 

@@ -1,0 +1,358 @@
+// libsck: the C-ABI of include/sck.h.
+//
+// Compiles the HIP source generated per system with hipRTC for gfx950 (cached as code objects on disk, keyed
+// by a hash of source + options and verified against the stored source), loads the module, resolves its
+// kernels, and executes the launch plans the host generator computed: coloured sweeps, the sequential chain,
+// the synchronous control, the float64 observables, and HIP graphs of k sweeps.
+#include "sck.h"
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(SCK_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+const char *kArch = "gfx950";
+
+std::vector<std::string> compile_options() {
+    return {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+            // packed fp32 instructions are left to the compiler's scalar path (no v_pk_*_f32)
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+}
+
+uint64_t fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull) {
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string cache_key(const std::string &src) {
+    std::string all = src;
+    for (auto &o : compile_options()) all += "\n//" + o;
+    all += "\n//arch=";
+    all += kArch;
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%016llx%08zx", (unsigned long long)fnv1a(all), all.size());
+    return buf;
+}
+
+bool read_file(const std::string &path, std::string &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+bool write_file_atomic(const std::string &path, const std::string &data) {
+    std::string tmp = path + ".tmp" + std::to_string((unsigned long long)fnv1a(data) & 0xffffff);
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return false;
+        f.write(data.data(), (std::streamsize)data.size());
+        if (!f) return false;
+    }
+    return std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+void mkdirs(const std::string &dir) {
+    std::string cur;
+    for (size_t i = 0; i < dir.size(); ++i) {
+        cur += dir[i];
+        if (dir[i] == '/' || i + 1 == dir.size()) ::mkdir(cur.c_str(), 0755);
+    }
+}
+
+int rtc_compile(const std::string &src, std::string &code) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "sck_system.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return fail(SCK_E_COMPILE, "hiprtcCreateProgram failed");
+    auto opts = compile_options();
+    std::vector<const char *> argv;
+    for (auto &o : opts) argv.push_back(o.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)argv.size(), argv.data());
+    size_t logsz = 0;
+    hiprtcGetProgramLogSize(prog, &logsz);
+    std::string log(logsz, '\0');
+    if (logsz) hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(SCK_E_COMPILE, std::string("hipRTC compilation failed: ") + hiprtcGetErrorString(r) + "\n" + log);
+    }
+    size_t sz = 0;
+    hiprtcGetCodeSize(prog, &sz);
+    code.assign(sz, '\0');
+    hiprtcGetCode(prog, &code[0]);
+    hiprtcDestroyProgram(&prog);
+    return SCK_OK;
+}
+
+// code object for `src`: from the cache directory when present (source verified), else compiled and stored
+int code_object(const std::string &src, const char *cache_dir, std::string &code) {
+    std::string key = cache_key(src);
+    std::string dir = cache_dir ? cache_dir : "";
+    if (!dir.empty()) {
+        std::string cached_src;
+        if (read_file(dir + "/" + key + ".hip", cached_src) && cached_src == src &&
+            read_file(dir + "/" + key + ".co", code) && !code.empty())
+            return SCK_OK;
+    }
+    int rc = rtc_compile(src, code);
+    if (rc) return rc;
+    if (!dir.empty()) {
+        mkdirs(dir);
+        write_file_atomic(dir + "/" + key + ".co", code);
+        write_file_atomic(dir + "/" + key + ".hip", src);
+    }
+    return SCK_OK;
+}
+
+}  // namespace
+
+struct sck_handle {
+    hipModule_t module = nullptr;
+    std::vector<hipFunction_t> fn;
+    std::vector<std::string> names;
+    std::vector<sck_launch> sweep, ref, sync, obs;
+    sck_launch bump{};
+    int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
+};
+
+struct sck_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+int launch(sck_handle *h, const sck_launch &l, const sck_args &a, hipStream_t s) {
+    if (l.kernel < 0 || l.kernel >= (int)h->fn.size()) return fail(SCK_E_INVALID, "launch plan names an unknown kernel");
+    sck_args copy = a;
+    void *params[] = {&copy};
+    HIP_TRY(hipModuleLaunchKernel(h->fn[l.kernel], l.grid[0], l.grid[1], l.grid[2], l.block[0], l.block[1],
+                                  l.block[2], l.shmem, s, params, nullptr));
+    return SCK_OK;
+}
+
+int bump(sck_handle *h, const sck_args &a, int64_t by, hipStream_t s) {
+    sck_args b = a;
+    b.sweep_offset = by;
+    return launch(h, h->bump, b, s);
+}
+
+int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_t hi, hipStream_t s) {
+    int ns = (int)h->sweep.size();
+    if (hi < 0 || hi > ns) hi = ns;
+    if (lo < 0 || lo > hi) return fail(SCK_E_INVALID, "class range out of bounds");
+    sck_args a = *args;
+    for (int32_t i = 0; i < n; ++i) {
+        a.sweep_offset = i;
+        for (int k = lo; k < hi; ++k) {
+            int rc = launch(h, h->sweep[k], a, s);
+            if (rc) return rc;
+        }
+    }
+    return n > 0 ? bump(h, *args, n, s) : SCK_OK;
+}
+
+int do_observe(sck_handle *h, const sck_args *args, hipStream_t s) {
+    for (auto &l : h->obs) {
+        int rc = launch(h, l, *args, s);
+        if (rc) return rc;
+    }
+    return SCK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sck_version(void) { return SCK_ABI_VERSION; }
+
+const char *sck_last_error(void) { return g_err.c_str(); }
+
+int sck_precompile(const char *hip_src, const char *cache_dir) {
+    if (!hip_src) return fail(SCK_E_INVALID, "null source");
+    std::string code;
+    return code_object(hip_src, cache_dir, code);
+}
+
+int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sck_handle **out) {
+    if (!d || !hip_src || !out) return fail(SCK_E_INVALID, "null argument");
+    if (d->abi_version != SCK_ABI_VERSION) return fail(SCK_E_INVALID, "descriptor ABI version mismatch");
+    if (d->n_kernels <= 0 || !d->kernel_names) return fail(SCK_E_INVALID, "no kernels");
+    std::string code;
+    int rc = code_object(hip_src, cache_dir, code);
+    if (rc) return rc;
+    auto *h = new sck_handle();
+    hipError_t e = hipModuleLoadData(&h->module, code.data());
+    if (e != hipSuccess) {
+        delete h;
+        return fail(SCK_E_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i < d->n_kernels; ++i) {
+        hipFunction_t f;
+        e = hipModuleGetFunction(&f, h->module, d->kernel_names[i]);
+        if (e != hipSuccess) {
+            hipModuleUnload(h->module);
+            delete h;
+            return fail(SCK_E_HIP, std::string("kernel ") + d->kernel_names[i] + ": " + hipGetErrorString(e));
+        }
+        h->fn.push_back(f);
+        h->names.push_back(d->kernel_names[i]);
+    }
+    h->sweep.assign(d->sweep, d->sweep + d->n_sweep);
+    h->ref.assign(d->ref, d->ref + d->n_ref);
+    h->sync.assign(d->sync, d->sync + d->n_sync);
+    h->obs.assign(d->obs, d->obs + d->n_obs);
+    h->bump = *d->bump;
+    h->mode = d->mode;
+    h->n = d->n;
+    h->b = d->b;
+    h->batch = d->batch;
+    h->words = d->words;
+    h->obs_chunks = d->obs_chunks;
+    *out = h;
+    return SCK_OK;
+}
+
+int sck_destroy(sck_handle *h) {
+    if (!h) return SCK_OK;
+    if (h->module) hipModuleUnload(h->module);
+    delete h;
+    return SCK_OK;
+}
+
+int sck_sweep(sck_handle *h, const sck_args *args, int32_t n, int32_t class_lo, int32_t class_hi, void *stream) {
+    if (!h || !args || n < 0) return fail(SCK_E_INVALID, "invalid sweep arguments");
+    return do_sweeps(h, args, n, class_lo, class_hi, (hipStream_t)stream);
+}
+
+int sck_reference_sweep(sck_handle *h, const sck_args *args, int32_t n, void *stream) {
+    if (!h || !args || n < 0) return fail(SCK_E_INVALID, "invalid reference-sweep arguments");
+    hipStream_t s = (hipStream_t)stream;
+    sck_args a = *args;
+    for (int32_t i = 0; i < n; ++i) {
+        a.sweep_offset = i;
+        for (auto &l : h->ref) {
+            int rc = launch(h, l, a, s);
+            if (rc) return rc;
+        }
+    }
+    return n > 0 ? bump(h, *args, n, s) : SCK_OK;
+}
+
+int sck_sync_sweep(sck_handle *h, const sck_args *args, int32_t n, int64_t spin_bytes, void *stream) {
+    if (!h || !args || n < 0 || !args->dst || spin_bytes <= 0) return fail(SCK_E_INVALID, "invalid sync arguments");
+    hipStream_t s = (hipStream_t)stream;
+    sck_args a = *args;
+    for (int32_t i = 0; i < n; ++i) {
+        a.sweep_offset = i;
+        HIP_TRY(hipMemcpyAsync(a.dst, a.spins, (size_t)spin_bytes, hipMemcpyDeviceToDevice, s));
+        for (auto &l : h->sync) {
+            int rc = launch(h, l, a, s);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(a.spins, a.dst, (size_t)spin_bytes, hipMemcpyDeviceToDevice, s));
+    }
+    return n > 0 ? bump(h, *args, n, s) : SCK_OK;
+}
+
+int sck_observe(sck_handle *h, const sck_args *args, void *stream) {
+    if (!h || !args || !args->obs || !args->obs_scratch) return fail(SCK_E_INVALID, "observe needs obs buffers");
+    return do_observe(h, args, (hipStream_t)stream);
+}
+
+int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t observe, sck_graph **out) {
+    if (!h || !args || k < 0 || !out) return fail(SCK_E_INVALID, "invalid graph arguments");
+    auto *g = new sck_graph();
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(SCK_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        hipStreamDestroy(g->stream);
+        delete g;
+        return fail(SCK_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    int rc = do_sweeps(h, args, k, 0, -1, g->stream);
+    if (!rc && observe) rc = do_observe(h, args, g->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(g->stream, &graph);
+    if (rc || e2 != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        hipStreamDestroy(g->stream);
+        delete g;
+        return rc ? rc : fail(SCK_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    }
+    g->graph = graph;
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        hipStreamDestroy(g->stream);
+        delete g;
+        return fail(SCK_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return SCK_OK;
+}
+
+int sck_graph_replay(sck_graph *g, void *stream) {
+    if (!g) return fail(SCK_E_INVALID, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return SCK_OK;
+}
+
+int sck_graph_destroy(sck_graph *g) {
+    if (!g) return SCK_OK;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return SCK_OK;
+}
+
+int sck_time_replays(sck_graph *g, int32_t reps, void *stream, float *ms) {
+    if (!g || reps <= 0 || !ms) return fail(SCK_E_INVALID, "invalid timing arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, s));
+    for (int32_t i = 0; i < reps; ++i) HIP_TRY(hipGraphLaunch(g->exec, s));
+    HIP_TRY(hipEventRecord(b, s));
+    HIP_TRY(hipEventSynchronize(b));
+    HIP_TRY(hipEventElapsedTime(ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return SCK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""ctypes binding of libsck.so (include/sck.h) and its in-tree build.
+
+torch is imported before the library is loaded so that libsck's libamdhip64.so.7 / libhiprtc.so.7 resolve
+to the copies torch already mapped: one HIP runtime per process, so torch's streams and pointers are valid
+handles for libsck.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede loading libsck: shared HIP runtime)
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent                      # spin-colour-kernel_amd/
+REPO = ROOT.parent
+INCLUDE = REPO / "include"
+LIB = PKG / "libsck.so"
+SOURCES = [ROOT / "csrc" / "sck.cpp"]
+CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
+
+SCK_ABI_VERSION = 1
+
+
+class Args(C.Structure):
+    _fields_ = [("spins", C.c_void_p), ("dst", C.c_void_p), ("mask", C.c_void_p), ("k1", C.c_void_p),
+                ("beta", C.c_void_p), ("table", C.c_void_p), ("state", C.c_void_p), ("width", C.c_void_p),
+                ("site_ku", C.c_void_p), ("site_axis", C.c_void_p), ("obs", C.c_void_p),
+                ("obs_scratch", C.c_void_p), ("sweep_offset", C.c_int64), ("nrep", C.c_int32),
+                ("flags", C.c_int32)]
+
+
+class LaunchC(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("grid", C.c_uint32 * 3), ("block", C.c_uint32 * 3), ("shmem", C.c_uint32)]
+
+
+class Desc(C.Structure):
+    _fields_ = [("abi_version", C.c_int32), ("mode", C.c_int32), ("n", C.c_int32), ("b", C.c_int32),
+                ("L1", C.c_int32), ("L2", C.c_int32), ("L3", C.c_int32), ("batch", C.c_int32),
+                ("words", C.c_int32), ("obs_chunks", C.c_int32),
+                ("kernel_names", C.POINTER(C.c_char_p)), ("n_kernels", C.c_int32),
+                ("sweep", C.POINTER(LaunchC)), ("n_sweep", C.c_int32),
+                ("ref", C.POINTER(LaunchC)), ("n_ref", C.c_int32),
+                ("sync", C.POINTER(LaunchC)), ("n_sync", C.c_int32),
+                ("obs", C.POINTER(LaunchC)), ("n_obs", C.c_int32),
+                ("bump", C.POINTER(LaunchC))]
+
+
+#: every symbol include/sck.h declares
+EXPORTS = ["sck_version", "sck_last_error", "sck_create", "sck_precompile", "sck_destroy", "sck_sweep",
+           "sck_reference_sweep", "sck_sync_sweep", "sck_observe", "sck_graph_capture", "sck_graph_replay",
+           "sck_graph_destroy", "sck_time_replays"]
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile libsck.so in-tree with hipcc (host code; device code is compiled per system by hipRTC)."""
+    cmd = ["hipcc", "-shared", "-fPIC", "-O2", "-std=c++17", f"-I{INCLUDE}", *map(str, SOURCES),
+           "-o", str(LIB) + ".tmp", "-lhiprtc"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building libsck.so failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(str(LIB) + ".tmp", LIB)
+    if verbose:
+        print(" ".join(cmd))
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libsck.so; raises when it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(str(LIB))
+        L.sck_last_error.restype = C.c_char_p
+        L.sck_version.restype = C.c_int
+        for name in EXPORTS:
+            getattr(L, name)
+        L.sck_create.argtypes = [C.POINTER(Desc), C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.sck_precompile.argtypes = [C.c_char_p, C.c_char_p]
+        L.sck_destroy.argtypes = [C.c_void_p]
+        L.sck_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        L.sck_reference_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_void_p]
+        L.sck_sync_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int64, C.c_void_p]
+        L.sck_observe.argtypes = [C.c_void_p, C.POINTER(Args), C.c_void_p]
+        L.sck_graph_capture.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
+        L.sck_graph_replay.argtypes = [C.c_void_p, C.c_void_p]
+        L.sck_graph_destroy.argtypes = [C.c_void_p]
+        L.sck_time_replays.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_float)]
+        if L.sck_version() != SCK_ABI_VERSION:
+            raise RuntimeError("libsck.so ABI version mismatch: rebuild it")
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "libsck") -> None:
+    if rc != 0:
+        msg = lib().sck_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def launches(ls) -> tuple:
+    arr = (LaunchC * max(1, len(ls)))()
+    for i, l in enumerate(ls):
+        arr[i].kernel = l.kernel
+        for k in range(3):
+            arr[i].grid[k] = int(l.grid[k])
+            arr[i].block[k] = int(l.block[k])
+        arr[i].shmem = int(l.shmem)
+    return arr, len(ls)
+
+
+def precompile(src: str, cache_dir: Path | None = None) -> None:
+    """Compile one generated module into the code-object cache (no GPU needed)."""
+    check(lib().sck_precompile(src.encode(), str(cache_dir or CACHE).encode()), "hipRTC precompile")

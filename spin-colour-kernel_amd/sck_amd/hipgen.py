@@ -1,0 +1,1051 @@
+"""Per-system HIP source generator for gfx950 (the MI355X counterpart of SCK/generator.py and SCK/bitwise.py).
+
+One System gives one HIP translation unit, compiled by libsck with hipRTC. Everything fixed at generation time
+is straight-line code: model, bond table grouped by J value, extra (DMI) matrices, field, anisotropies,
+colouring and box. Runtime values are the replica index, beta, the acceptance table and the Philox state.
+
+Kernels of a module (all take one `sck_args` by value):
+
+  sck_class_<k>   one (colour, sublattice) class of the coloured sweep, k in sweep order. Lanes cover the class
+                  sites of one row after another (the colouring's x-period P gives each row's sites as
+                  x0 + P j), one block row per replica, so loads along x stay coalesced in the reference
+                  layout S[B, n, b, L3, L2, L1].
+  sck_seq         the sequential chain: one thread per replica (per word for packed/bitwise storage) visiting
+                  the classes in sweep order and their sites in raster order (SCK/sampler.py:281-293).
+  sck_sync_<a>    synchronous control of sublattice a (SCK/sampler.py:271-279).
+  sck_obs, sck_obs_final   float64 energy and magnetization: per-block partial sums, then a fixed-order sum.
+  sck_bump        state[1] += sweep_offset.
+
+The per-site update follows SCK/generator.py:202-318 operation by operation in fp32 with contraction off, so
+the sequential kernel, the class kernels and the CPU oracle agree:
+
+  * Ising (n = 1): the local field only enters through dE = 2 s v, so the kernel sums each J group's
+    neighbour spins as integers and reads exp(-beta dE) (Barker: 1 + exp(beta dE)) from a per-replica table
+    built on the host with numpy float32 exp, exactly the values the reference's CPU chain computes. The
+    draw is skipped where the table decides alone (p >= 1 or p == 0): the Philox stream is counter-based, so
+    skipping a draw changes no other draw.
+  * XY / Heisenberg: fp32 local field, proposal and dE in the reference's order; cos/sin/exp are the device
+    library's accurate versions (parity: decision tolerance).
+
+Philox-4x32-10: key (seed, 0), counter (rep * nsite + own) split into (low, high) words -- for counters below
+2^31 exactly the reference's int32 counter (SCK/generator.py:275-276), above it the high word Triton's 64-bit
+offsets would use (triton/language/random.py:102-110).
+"""
+
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .system import System
+
+__all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT8", "MODE_F32", "MODE_PACKED",
+           "MODE_BITWISE", "MODE_BITWISE_INDEP", "BLOCK"]
+
+MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP = range(5)
+BLOCK = 256
+WORD = 32
+GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
+LEVELS = 32
+
+
+@dataclass
+class Launch:
+    kernel: int
+    grid: tuple[int, int, int]
+    block: tuple[int, int, int] = (BLOCK, 1, 1)
+    shmem: int = 0
+
+
+@dataclass
+class Module:
+    src: str
+    kernels: list[str]
+    sweep: list[Launch]
+    ref: list[Launch]
+    sync: list[Launch]
+    obs: list[Launch]
+    bump: Launch
+    obs_chunks: int
+    classes: list[tuple[int, int]]
+    table_entries: int = 0            # Ising: table entries per replica
+    ising_layout: list = field(default_factory=list)
+    key: str = ""
+
+
+def _hexf(v: float) -> str:
+    """A constant as the fp32 value the reference bakes into its source (SCK/generator.py:44-45), written as an
+    exact C99 hexadecimal literal."""
+    return f"{float(np.float32(v)).hex()}f"
+
+
+def _f64(x: float) -> str:
+    return float(x).hex()
+
+
+def x_period(col) -> int:
+    """Period along x of a linear colouring: lcm over factors of m_r / gcd(K_r1, m_r) (SCK/generator.py:58-63)."""
+    P = 1
+    for r, mr in enumerate(col.m):
+        P = math.lcm(P, mr // math.gcd(int(col.K[r][0]) % mr or mr, mr))
+    return P
+
+
+# --------------------------------------------------------------------------------------- bond plans
+def _groups(sys_: System, a: int):
+    """Bonds of sublattice a grouped by J value in first-appearance order, and the extras in bond order
+    (SCK/generator.py:222-233)."""
+    groups: dict[float, list[int]] = {}
+    extras: list[int] = []
+    for i, bd in enumerate(sys_.crystal.bonds):
+        if bd.alpha != a:
+            continue
+        groups.setdefault(float(sys_.J.get(bd.shell, 0.0)), []).append(i)
+        if i in sys_.extra:
+            extras.append(i)
+    out = []
+    for Jv, members in groups.items():
+        if Jv == 0.0 and not any(i in sys_.extra for i in members):
+            continue
+        out.append((Jv, members))
+    return out, extras
+
+
+def ising_layout(sys_: System):
+    """Digits of the Ising table index per sublattice: [(kind, size, payload)], kind 'g' (a J group's
+    neighbour sum + n), 'e' (an extra bond's neighbour value + 1) or 's' (own spin > 0). Returns
+    (per-sublattice digit lists, per-sublattice offsets, entries per replica)."""
+    layouts, offsets, total = [], [], 0
+    for a in range(sys_.b):
+        groups, extras = _groups(sys_, a)
+        digits = []
+        for Jv, members in groups:
+            if Jv != 0.0:
+                digits.append(("g", 2 * len(members) + 1, (Jv, members)))
+        for i in extras:
+            digits.append(("e", 3, i))
+        digits.append(("s", 2, None))
+        size = int(np.prod([d[1] for d in digits]))
+        layouts.append(digits)
+        offsets.append(total)
+        total += size
+    return layouts, offsets, total
+
+
+def ising_dE_values(sys_: System) -> np.ndarray:
+    """dE of every Ising table entry, float32, evaluated in the reference's fp32 operation order: groups in
+    order, v = v + fp32(J) * g, extras v = v + fp32(E) * t, field v = v + fp32(h), dE = -((t0 - s0) * v)
+    with t0 = -s0 (SCK/generator.py:226-296)."""
+    layouts, offsets, total = ising_layout(sys_)
+    out = np.zeros(total, np.float32)
+    f32 = np.float32
+    for a in range(sys_.b):
+        digits = layouts[a]
+        sizes = [d[1] for d in digits]
+        n_ent = int(np.prod(sizes))
+        idx = np.arange(n_ent)
+        vals = []
+        for sz in reversed(sizes):               # last digit fastest
+            vals.append(idx % sz)
+            idx = idx // sz
+        vals = vals[::-1]
+        groups, extras = _groups(sys_, a)
+        v = np.zeros(n_ent, np.float32)
+        di = 0
+        for Jv, members in groups:
+            if Jv != 0.0:
+                g = (vals[di] - len(members)).astype(np.float32)
+                di += 1
+                v = (v + f32(Jv) * g).astype(np.float32)
+        ext_vals = []
+        for i in extras:
+            ext_vals.append((vals[di] - 1).astype(np.float32))
+            di += 1
+        for j, i in enumerate(extras):
+            E = np.asarray(sys_.extra[i], np.float64).reshape(1, 1)
+            if E[0, 0] != 0.0:
+                v = (v + f32(E[0, 0]) * ext_vals[j]).astype(np.float32)
+        if sys_.h is not None and sys_.h[0] != 0.0:
+            v = (v + f32(sys_.h[0])).astype(np.float32)
+        s0 = np.where(vals[di] > 0, f32(1.0), f32(-1.0)).astype(np.float32)
+        t0 = (-s0).astype(np.float32)
+        dE = (-((t0 - s0).astype(np.float32) * v).astype(np.float32)).astype(np.float32)
+        out[offsets[a]:offsets[a] + n_ent] = dE
+    return out
+
+
+def ising_table(sys_: System, beta: np.ndarray) -> np.ndarray:
+    """Per-replica acceptance table [Bp, entries], float32: Metropolis p = exp((-beta) dE), Barker
+    q = 1 + exp(beta dE), with numpy's float32 exp, as the reference's CPU chain evaluates
+    `u0 < exp(-beta dE)` and `u0 * (1 + exp(beta dE)) < 1` (SCK/generator.py:303-309)."""
+    dE = ising_dE_values(sys_)
+    beta = np.asarray(beta, np.float32).reshape(-1, 1)
+    with np.errstate(over="ignore"):
+        if sys_.acceptance == "barker":
+            x = (beta * dE[None, :]).astype(np.float32)
+            return (np.float32(1.0) + np.exp(x)).astype(np.float32)
+        x = ((-beta) * dE[None, :]).astype(np.float32)
+        return np.exp(x).astype(np.float32)
+
+
+# --------------------------------------------------------------------------------------------- source
+HEADER = r"""
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef struct sck_args {
+    void *spins; void *dst; const int8_t *mask; const float *k1; const float *beta; const void *table;
+    const int64_t *state; const float *width; const float *site_ku; const float *site_axis;
+    double *obs; double *obs_scratch; int64_t sweep_offset; int32_t nrep; int32_t flags;
+} sck_args;
+
+#define DEV static __device__ __forceinline__
+
+// Philox-4x32-10 (Salmon et al., SC'11): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
+// 0x9E3779B9 / 0xBB67AE85; one 32x32->64 multiply gives each product's high and low words.
+DEV void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint64_t p2 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint32_t n0 = (uint32_t)(p0 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p2 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p0;
+        c3 = (uint32_t)p2;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// uniform in [0, 1): fp32(x < 0 ? ~x : x) * fp32(4.6566127342e-10), x the output read as int32
+DEV float u01(uint32_t x) {
+    int32_t v = (int32_t)x;
+    v = v < 0 ? ~v : v;
+    return (float)v * 4.6566127342e-10f;
+}
+
+// per-sweep Philox key (base_seed * 1000003 + sweep * 7919 + 12345) mod 2147483629
+DEV uint32_t sweep_key(const sck_args &A) {
+    const int64_t base = A.state[0];
+    const int64_t sweep = A.state[1] + A.sweep_offset;
+    return (uint32_t)((base * 1000003LL + sweep * 7919LL + 12345LL) % 2147483629LL);
+}
+
+DEV int wrap_add(int v, int d, int L) {
+    v += d;
+    v = v >= L ? v - L : v;
+    return v < 0 ? v + L : v;
+}
+"""
+
+
+class _Gen:
+    def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None):
+        self.s = sys_
+        self.mode = mode
+        self.B = int(B)
+        self.proposal = proposal
+        self.lm = learned_meta
+        self.n, self.b = sys_.n, sys_.b
+        self.L1, self.L2, self.L3 = sys_.L1, sys_.L2, sys_.L3
+        self.D = sys_.crystal.D
+        self.per = list(sys_.periodic) + [False] * (3 - self.D)
+        self.PL = self.L3 * self.L2 * self.L1
+        self.NS = self.b * self.PL
+        self.ROWS = self.L3 * self.L2
+        self.W = -(-self.B // WORD) if mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
+        self.Bp = self.W * WORD if mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
+        self.has_mask = sys_.mask is not None
+        self.classes = sys_.classes()
+        self.P = x_period(sys_.colouring)
+        self.BX = -(-self.L1 // self.P)
+        self.wide = self.Bp * self.NS >= 2 ** 31    # 64-bit Philox counter arithmetic needed
+        self.lines: list[str] = []
+
+    # -------------------------------------------------------------------------------------- helpers
+    def w(self, s: str = "") -> None:
+        self.lines.append(s)
+
+    def coord(self, var: str, d: int, L: int, periodic: bool):
+        """(expression, validity or None) of a neighbour coordinate (SCK/generator.py:48-55)."""
+        if d == 0:
+            return var, None
+        if periodic:
+            if abs(d) < L:
+                return f"wrap_add({var}, {d}, {L})", None
+            return f"((({var} + {d}) % {L} + {L}) % {L})", None
+        e = f"({var} + {d})"
+        return e, f"({e} >= 0 && {e} < {L})"
+
+    def nb_index(self, i: int):
+        """Code computing q (neighbour site index) and ok (inside the box) of bond i from x, y, z."""
+        bd = self.s.crystal.bonds[i]
+        d = list(bd.delta) + [0] * (3 - self.D)
+        ex, vx = self.coord("x", d[0], self.L1, self.per[0])
+        ey, vy = self.coord("y", d[1], self.L2, self.per[1])
+        ez, vz = self.coord("z", d[2], self.L3, self.per[2])
+        valid = " && ".join(p for p in (vx, vy, vz) if p) or "true"
+        return f"{bd.alpha2 * self.PL} + (({ez}) * {self.L2} + ({ey})) * {self.L1} + ({ex})", valid
+
+    def colour_expr(self, xv: str, a: int) -> str:
+        """Colour of site (xv, y, z) of sublattice a (mixed radix of gamma(c, alpha))."""
+        col = self.s.colouring
+        if not col.m:
+            return "0"
+        expr = None
+        for r, mr in enumerate(col.m):
+            K = list(col.K[r]) + [0] * (3 - self.D)
+            term = f"((({K[0]} * {xv} + {K[1]} * y + {K[2]} * z + {col.o[a][r]}) % {mr}))"
+            expr = term if expr is None else f"(({expr}) * {mr} + {term})"
+        return expr
+
+    # ------------------------------------------------------------------------------- site update
+    def emit_update_plain(self, a: int) -> None:
+        """__device__ update of one site of sublattice a for one replica (unpacked storage).
+        Returns 1 when the move is accepted. `src` is read, `dst` written (in place: src == dst)."""
+        s, n = self.s, self.n
+        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        w = self.w
+        w(f"DEV int upd_{a}(const sck_args &A, const {T} *__restrict__ src, {T} *__restrict__ dst, int64_t rbase, "
+          f"int rep, int own, int x, int y, int z, uint32_t key) {{")
+        groups, extras = _groups(s, a)
+        if n == 1:
+            layouts, offsets, total = ising_layout(s)
+            digits = layouts[a]
+            w("    int idx = 0;")
+            w(f"    const int s0 = (int)src[rbase + own];")
+            gi = 0
+            slot = {}
+            for Jv, members in groups:
+                w("    {")
+                w("        int g = 0;")
+                for i in members:
+                    q, ok = self.nb_index(i)
+                    w(f"        {{ const int q = {q}; const int t = ({ok}) ? (int)src[rbase + q] : 0; g += t;"
+                      + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
+                if Jv != 0.0:
+                    w(f"        gsum{gi} = g + {len(members)};")
+                    slot[gi] = True
+                    gi += 1
+                w("    }")
+            # declare before use: rewrite with declarations up front
+            decl = [f"    int gsum{k} = 0;" for k in range(gi)] + [f"    int ext{k} = 0;" for k in range(len(extras))]
+            # insert declarations after the s0 line
+            pos = len(self.lines) - 1
+            while not self.lines[pos].startswith("    const int s0"):
+                pos -= 1
+            self.lines[pos + 1:pos + 1] = decl
+            # index: digits in order, last fastest
+            terms = []
+            gk = 0
+            ek = 0
+            for kind, size, payload in digits:
+                if kind == "g":
+                    terms.append((f"gsum{gk}", size))
+                    gk += 1
+                elif kind == "e":
+                    terms.append((f"(ext{ek} + 1)", size))
+                    ek += 1
+                else:
+                    terms.append(("(s0 > 0 ? 1 : 0)", size))
+            expr = "0"
+            for t, size in terms:
+                expr = f"({expr}) * {size} + {t}"
+            w(f"    idx = {offsets[a]} + {expr};")
+            w(f"    const float p = ((const float *)A.table)[(int64_t)rep * {total} + idx];")
+            barker = s.acceptance == "barker"
+            if barker:
+                w("    int acc;")
+                w("    if (p == 1.0f) acc = 1;")               # u0 * 1 < 1 for every u0 < 1
+                w("    else if (!(p < __builtin_inff())) acc = 0;")   # u0 * inf is inf or NaN: never below 1
+                w("    else {")
+            else:
+                w("    int acc;")
+                w("    if (p >= 1.0f) acc = 1;")               # every u0 <= 1 - 2^-24 is below p
+                w("    else if (!(p > 0.0f)) acc = 0;")        # u0 >= 0 is never below 0
+                w("    else {")
+            self.emit_draw("        ")
+            if barker:
+                w("        acc = (u0 * p) < 1.0f;")
+            else:
+                w("        acc = u0 < p;")
+            w("    }")
+            w(f"    if (acc) dst[rbase + own] = ({T})(-s0);")
+            w("    return acc;")
+            w("}")
+            return
+        # vector models: fp32 local field in the reference's order
+        for k in range(n):
+            w(f"    float s{k} = (float)src[rbase + {k * self.NS} + own];")
+        for k in range(n):
+            w(f"    float v{k} = 0.0f;")
+        for j in range(len(extras)):
+            for k in range(n):
+                w(f"    float n{j}_{k} = 0.0f;")
+        for Jv, members in groups:
+            w("    {")
+            for k in range(n):
+                w(f"        float g{k} = 0.0f;")
+            for i in members:
+                q, ok = self.nb_index(i)
+                w(f"        {{ const int q = {q}; const bool ok = {ok};")
+                for k in range(n):
+                    w(f"          const float t{k} = ok ? (float)src[rbase + {k * self.NS} + q] : 0.0f; g{k} = g{k} + t{k};")
+                    if i in extras:
+                        w(f"          n{extras.index(i)}_{k} = t{k};")
+                w("        }")
+            if Jv != 0.0:
+                for k in range(n):
+                    w(f"        v{k} = v{k} + {_hexf(Jv)} * g{k};")
+            w("    }")
+        for j, i in enumerate(extras):
+            E = np.asarray(s.extra[i], np.float64)
+            for k in range(n):
+                terms = [f"{_hexf(E[k, l])} * n{j}_{l}" for l in range(n) if E[k, l] != 0.0]
+                if terms:
+                    acc = terms[0]
+                    for t in terms[1:]:
+                        acc = f"({acc}) + {t}"
+                    w(f"    v{k} = v{k} + ({acc});")
+        if s.h is not None:
+            for k in range(n):
+                if s.h[k] != 0.0:
+                    w(f"    v{k} = v{k} + {_hexf(s.h[k])};")
+        self.emit_draw("    ")
+        self.emit_proposal()
+        dsum = f"(t0 - s0) * v0"
+        for k in range(1, n):
+            dsum = f"({dsum}) + (t{k} - s{k}) * v{k}"
+        w(f"    float dE = -({dsum});")
+        if s.cubic_k1 is not None:
+            k2 = _hexf(s.cubic_k2)
+            w("    { const float k1 = A.k1[own];")
+            for p in ("t", "s"):
+                w(f"      const float a_{p} = k1 * ((({p}0 * {p}1) * ({p}0 * {p}1) + ({p}0 * {p}2) * ({p}0 * {p}2)) "
+                  f"+ ({p}1 * {p}2) * ({p}1 * {p}2)) + ({k2} * (({p}0 * {p}1) * {p}2)) * (({p}0 * {p}1) * {p}2);")
+            w("      dE = dE + (a_t - a_s); }")
+        if s.uniaxial is not None:
+            Du, e = s.uniaxial
+            se = " + ".join(f"s{k} * {_hexf(e[k])}" for k in range(n))
+            te = " + ".join(f"t{k} * {_hexf(e[k])}" for k in range(n))
+            w(f"    {{ const float se = {self._lassoc([f's{k} * {_hexf(e[k])}' for k in range(n)])};"
+              f" const float te = {self._lassoc([f't{k} * {_hexf(e[k])}' for k in range(n)])};")
+            w(f"      dE = dE + {_hexf(Du)} * (se * se - te * te); }}")
+            del se, te
+        if s.site_ku is not None:
+            w(f"    {{ const float ku = A.site_ku[own]; const float *ax = A.site_axis + (int64_t)own * {n};")
+            w(f"      const float sn = {self._lassoc([f's{k} * ax[{k}]' for k in range(n)])};"
+              f" const float tn = {self._lassoc([f't{k} * ax[{k}]' for k in range(n)])};")
+            w("      dE = dE + ku * (sn * sn - tn * tn); }")
+        w("    const float beta = A.beta[rep];")
+        if s.acceptance == "barker":
+            w("    const int acc = (u0 * (1.0f + expf(beta * dE))) < 1.0f;")
+        else:
+            w("    const int acc = u0 < expf((-beta) * dE);")
+        w("    if (acc) {")
+        for k in range(n):
+            w(f"        dst[rbase + {k * self.NS} + own] = t{k};")
+        w("    }")
+        w("    return acc;")
+        w("}")
+
+    @staticmethod
+    def _lassoc(terms):
+        acc = terms[0]
+        for t in terms[1:]:
+            acc = f"({acc}) + {t}"
+        return acc
+
+    def emit_draw(self, ind: str) -> None:
+        w = self.w
+        if self.wide:
+            w(f"{ind}const uint64_t ctr = (uint64_t)rep * {self.NS}ull + (uint64_t)own;")
+            w(f"{ind}uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = 0u, c3 = 0u;")
+        else:
+            w(f"{ind}uint32_t c0 = (uint32_t)rep * {self.NS}u + (uint32_t)own, c1 = 0u, c2 = 0u, c3 = 0u;")
+        w(f"{ind}philox10(c0, c1, c2, c3, key, 0u);")
+        w(f"{ind}const float u0 = u01(c0);")
+        if self.n > 1:
+            w(f"{ind}const float u1 = u01(c1);")
+            w(f"{ind}const float u2 = u01(c2);")
+
+    def emit_proposal(self) -> None:
+        """Trial spin t from (u1, u2): uniform redraw (SCK/generator.py:281-289) or a cone of half-angle w."""
+        w, n = self.w, self.n
+        TWO_PI = "6.283185307179586f"
+        if self.proposal == "uniform":
+            if n == 2:
+                w(f"    const float phi = {TWO_PI} * u1;")
+                w("    float t0, t1; sincosf(phi, &t1, &t0);")
+            else:
+                w("    const float cz = 2.0f * u1 - 1.0f;")
+                w(f"    const float phi = {TWO_PI} * u2;")
+                w("    const float rr = sqrtf(fmaxf(1.0f - cz * cz, 0.0f));")
+                w("    float sp, cp; sincosf(phi, &sp, &cp);")
+                w("    const float t0 = rr * cp, t1 = rr * sp, t2 = cz;")
+            return
+        # cone proposal
+        if self.proposal == "cone":
+            w("    const float wd = A.width[rep];")
+        else:
+            lm = self.lm
+            vv = self._lassoc([f"v{k} * v{k}" for k in range(n)])
+            w(f"    const float hv = sqrtf({vv});")
+            w("    const float f0 = hv * A.beta[rep];")
+            ve = self._lassoc([f"v{k} * {_hexf(lm['axis'][k])}" for k in range(n)])
+            w(f"    const float f2 = hv > 0.0f ? fabsf({ve}) / hv : 0.0f;")
+            w(f"    int i0 = (int)(f0 * {_hexf(lm['s0'])}); i0 = i0 > {lm['n0'] - 1} ? {lm['n0'] - 1} : (i0 < 0 ? 0 : i0);")
+            w(f"    int i1 = (int)(f2 * {_hexf(lm['s1'])}); i1 = i1 > {lm['n1'] - 1} ? {lm['n1'] - 1} : (i1 < 0 ? 0 : i1);")
+            w(f"    const float wd = A.width[((int64_t)rep * {lm['n0']} + i0) * {lm['n1']} + i1];")
+        if n == 2:
+            w("    const float ph = atan2f(s1, s0) + wd * (2.0f * u1 - 1.0f);")
+            w("    float t0, t1; sincosf(ph, &t1, &t0);")
+        else:
+            w("    const float cc = 1.0f - u1 * (1.0f - cosf(wd));")
+            w("    const float st = sqrtf(fmaxf(1.0f - cc * cc, 0.0f));")
+            w(f"    const float phi = {TWO_PI} * u2;")
+            w("    const bool nz = fabsf(s2) > 0.9f;")
+            w("    const float ax0 = nz ? 1.0f : 0.0f, ax1 = 0.0f, ax2 = nz ? 0.0f : 1.0f;")
+            w("    float e10 = ax1 * s2 - ax2 * s1, e11 = ax2 * s0 - ax0 * s2, e12 = ax0 * s1 - ax1 * s0;")
+            w("    const float ne = sqrtf(e10 * e10 + e11 * e11 + e12 * e12);")
+            w("    e10 = e10 / ne; e11 = e11 / ne; e12 = e12 / ne;")
+            w("    const float e20 = s1 * e12 - s2 * e11, e21 = s2 * e10 - s0 * e12, e22 = s0 * e11 - s1 * e10;")
+            w("    float sp, cp; sincosf(phi, &sp, &cp);")
+            w("    float t0 = cc * s0 + st * (cp * e10 + sp * e20);")
+            w("    float t1 = cc * s1 + st * (cp * e11 + sp * e21);")
+            w("    float t2 = cc * s2 + st * (cp * e12 + sp * e22);")
+            w("    const float nt = sqrtf(t0 * t0 + t1 * t1 + t2 * t2);")
+            w("    t0 = t0 / nt; t1 = t1 / nt; t2 = t2 / nt;")
+
+    # ------------------------------------------------------------------------------- plain kernels
+    def emit_plain(self) -> None:
+        w = self.w
+        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        for a in range(self.b):
+            self.emit_update_plain(a)
+            w()
+        # class kernels
+        for k, (c, a) in enumerate(self.classes):
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
+            w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
+            w(f"    const int row = t / {self.BX};")
+            w(f"    if (row >= {self.ROWS}) return;")
+            w(f"    const int j = t - row * {self.BX};")
+            w(f"    const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+            self.emit_residue(c, a)
+            w(f"    const int x = x0 + {self.P} * j;")
+            w(f"    if (x >= {self.L1}) return;")
+            w(f"    const int own = {a * self.PL} + row * {self.L1} + x;")
+            if self.has_mask:
+                w("    if (!A.mask[own]) return;")
+            w("    const int rep = blockIdx.y + blockIdx.z * gridDim.y;")
+            w("    if (rep >= A.nrep) return;")
+            w("    const uint32_t key = sweep_key(A);")
+            w(f"    {T} *S = ({T} *)A.spins;")
+            w(f"    upd_{a}(A, S, S, (int64_t)rep * {self.n * self.NS}, rep, own, x, y, z, key);")
+            w("}")
+            w()
+        # sequential chain: one thread per replica
+        w(f'extern "C" __global__ void __launch_bounds__(64) sck_seq(sck_args A) {{')
+        w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
+        w("    if (rep >= A.nrep) return;")
+        w("    const uint32_t key = sweep_key(A);")
+        w(f"    {T} *S = ({T} *)A.spins;")
+        w(f"    const int64_t rbase = (int64_t)rep * {self.n * self.NS};")
+        for c, a in self.classes:
+            w(f"    for (int z = 0; z < {self.L3}; ++z) for (int y = 0; y < {self.L2}; ++y) for (int x = 0; x < {self.L1}; ++x) {{")
+            w(f"        if ({self.colour_expr('x', a)} != {c}) continue;")
+            w(f"        const int own = {a * self.PL} + (z * {self.L2} + y) * {self.L1} + x;")
+            if self.has_mask:
+                w("        if (!A.mask[own]) continue;")
+            w(f"        upd_{a}(A, S, S, rbase, rep, own, x, y, z, key);")
+            w("    }")
+        w("}")
+        w()
+        # synchronous control: every site of sublattice a from the old state
+        for a in range(self.b):
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_sync_{a}(sck_args A) {{')
+            w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
+            w(f"    if (t >= {self.PL}) return;")
+            w(f"    const int x = t % {self.L1}, row = t / {self.L1};")
+            w(f"    const int z = row / {self.L2}, y = row % {self.L2};")
+            w(f"    const int own = {a * self.PL} + t;")
+            if self.has_mask:
+                w("    if (!A.mask[own]) return;")
+            w("    const int rep = blockIdx.y + blockIdx.z * gridDim.y;")
+            w("    if (rep >= A.nrep) return;")
+            w("    const uint32_t key = sweep_key(A);")
+            w(f"    const int64_t rbase = (int64_t)rep * {self.n * self.NS};")
+            w(f"    upd_{a}(A, (const {T} *)A.spins, ({T} *)A.dst, rbase, rep, own, x, y, z, key);")
+            w("}")
+            w()
+
+    def emit_residue(self, c: int, a: int) -> None:
+        """x0: the x residue in [0, P) of row (y, z) where sublattice a has colour c (P when none)."""
+        w = self.w
+        w(f"    int x0 = {self.P};")
+        for q in range(self.P - 1, -1, -1):
+            w(f"    if ({self.colour_expr(str(q), a)} == {c}) x0 = {q};")
+        w(f"    if (x0 >= {self.P}) return;")
+
+    # ------------------------------------------------------------------------------ packed kernels
+    def emit_packed(self) -> None:
+        """Packed Ising (SCK/generator.py:157-161, 246-253, 306-308): word w of a site holds replicas
+        32w .. 32w+31. Each group of 32 lanes computes the 32 replicas of one (word, site), each replica with
+        its own Philox counter and beta, and the wave's ballot packs the new bits."""
+        w = self.w
+        s = self.s
+        layouts, offsets, total = ising_layout(s)
+        barker = s.acceptance == "barker"
+        for a in range(self.b):
+            groups, extras = _groups(s, a)
+            w(f"DEV int dec_{a}(const sck_args &A, const int32_t *__restrict__ src, int64_t wbase, int lane, int rep, "
+              f"int own, int x, int y, int z, uint32_t key, int *s_out) {{")
+            w("    const int s0 = ((src[wbase + own] >> lane) & 1) ? 1 : -1;")
+            w("    *s_out = s0;")
+            gi = 0
+            decl_pos = len(self.lines)
+            for Jv, members in groups:
+                w("    {")
+                w("        int g = 0;")
+                for i in members:
+                    q, ok = self.nb_index(i)
+                    mk = " && A.mask[q]" if self.has_mask else ""
+                    w(f"        {{ const int q = {q}; int t = 0; if (({ok}){mk}) t = ((src[wbase + q] >> lane) & 1) ? 1 : -1; g += t;"
+                      + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
+                if Jv != 0.0:
+                    w(f"        gsum{gi} = g + {len(members)};")
+                    gi += 1
+                w("    }")
+            decl = [f"    int gsum{k} = 0;" for k in range(gi)] + [f"    int ext{k} = 0;" for k in range(len(extras))]
+            self.lines[decl_pos:decl_pos] = decl
+            terms, gk, ek = [], 0, 0
+            for kind, size, payload in layouts[a]:
+                if kind == "g":
+                    terms.append((f"gsum{gk}", size)); gk += 1
+                elif kind == "e":
+                    terms.append((f"(ext{ek} + 1)", size)); ek += 1
+                else:
+                    terms.append(("(s0 > 0 ? 1 : 0)", size))
+            expr = "0"
+            for t, size in terms:
+                expr = f"({expr}) * {size} + {t}"
+            w(f"    const float p = ((const float *)A.table)[(int64_t)rep * {total} + {offsets[a]} + {expr}];")
+            if barker:
+                w("    if (p == 1.0f) return 1;")
+                w("    if (!(p < __builtin_inff())) return 0;")
+            else:
+                w("    if (p >= 1.0f) return 1;")
+                w("    if (!(p > 0.0f)) return 0;")
+            self.emit_draw("    ")
+            w("    return " + ("(u0 * p) < 1.0f;" if barker else "u0 < p;"))
+            w("}")
+            w()
+        nsite_w = self.NS
+        for k, (c, a) in enumerate(self.classes):
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
+            w("    const int lane = threadIdx.x & 31;")
+            w("    const int t = blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5);")
+            w("    const int wd = blockIdx.y;")
+            w(f"    const int row = t / {self.BX};")
+            w(f"    const int j = t - row * {self.BX};")
+            w(f"    const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+            w(f"    bool live = row < {self.ROWS};")
+            w(f"    int x0 = {self.P};")
+            for q in range(self.P - 1, -1, -1):
+                w(f"    if ({self.colour_expr(str(q), a)} == {c}) x0 = {q};")
+            w(f"    const int x = x0 + {self.P} * j;")
+            w(f"    live = live && x0 < {self.P} && x < {self.L1};")
+            w(f"    const int own = live ? {a * self.PL} + row * {self.L1} + x : 0;")
+            if self.has_mask:
+                w("    live = live && A.mask[own];")
+            w("    const uint32_t key = sweep_key(A);")
+            w("    int32_t *S = (int32_t *)A.spins;")
+            w(f"    const int64_t wbase = (int64_t)wd * {nsite_w};")
+            w("    const int rep = wd * 32 + lane;")
+            w("    int s0 = 1, acc = 0;")
+            w(f"    if (live) acc = dec_{a}(A, S, wbase, lane, rep, own, x, y, z, key, &s0);")
+            w("    const int nb = acc ? (s0 < 0) : (s0 > 0);")
+            w("    const unsigned long long bal = __ballot(nb);")
+            w("    if (live && lane == 0) S[wbase + own] = (int32_t)(uint32_t)(bal >> (threadIdx.x & 32));")
+            w("}")
+            w()
+        # sequential: one thread per word, replicas of a word one after another at each site
+        w(f'extern "C" __global__ void __launch_bounds__(64) sck_seq(sck_args A) {{')
+        w("    const int wd = blockIdx.x * blockDim.x + threadIdx.x;")
+        w("    if (wd >= A.nrep) return;")
+        w("    const uint32_t key = sweep_key(A);")
+        w("    int32_t *S = (int32_t *)A.spins;")
+        w(f"    const int64_t wbase = (int64_t)wd * {nsite_w};")
+        for c, a in self.classes:
+            w(f"    for (int z = 0; z < {self.L3}; ++z) for (int y = 0; y < {self.L2}; ++y) for (int x = 0; x < {self.L1}; ++x) {{")
+            w(f"        if ({self.colour_expr('x', a)} != {c}) continue;")
+            w(f"        const int own = {a * self.PL} + (z * {self.L2} + y) * {self.L1} + x;")
+            if self.has_mask:
+                w("        if (!A.mask[own]) continue;")
+            w("        uint32_t word = 0u;")
+            w("        for (int lane = 0; lane < 32; ++lane) {")
+            w(f"            int s0; const int acc = dec_{a}(A, S, wbase, lane, wd * 32 + lane, own, x, y, z, key, &s0);")
+            w("            const int nb = acc ? (s0 < 0) : (s0 > 0);")
+            w("            word |= (uint32_t)nb << lane;")
+            w("        }")
+            w("        S[wbase + own] = (int32_t)word;")
+            w("    }")
+        w("}")
+        w()
+        for a in range(self.b):
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_sync_{a}(sck_args A) {{')
+            w("    const int lane = threadIdx.x & 31;")
+            w("    const int t = blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5);")
+            w("    const int wd = blockIdx.y;")
+            w(f"    bool live = t < {self.PL};")
+            w(f"    const int tt = live ? t : 0;")
+            w(f"    const int x = tt % {self.L1}, row = tt / {self.L1};")
+            w(f"    const int z = row / {self.L2}, y = row % {self.L2};")
+            w(f"    const int own = {a * self.PL} + tt;")
+            if self.has_mask:
+                w("    live = live && A.mask[own];")
+            w("    const uint32_t key = sweep_key(A);")
+            w(f"    const int64_t wbase = (int64_t)wd * {nsite_w};")
+            w("    int s0 = 1, acc = 0;")
+            w(f"    if (live) acc = dec_{a}(A, (const int32_t *)A.spins, wbase, lane, wd * 32 + lane, own, x, y, z, key, &s0);")
+            w("    const int nb = acc ? (s0 < 0) : (s0 > 0);")
+            w("    const unsigned long long bal = __ballot(nb);")
+            w("    if (live && lane == 0) ((int32_t *)A.dst)[wbase + own] = (int32_t)(uint32_t)(bal >> (threadIdx.x & 32));")
+            w("}")
+            w()
+
+    # ----------------------------------------------------------------------------- bitwise kernels
+    def bitwise_params(self):
+        s = self.s
+        Js = {float(s.J.get(bd.shell, 0.0)) for bd in s.crystal.bonds} - {0.0}
+        h = float(s.h[0]) if s.h is not None else 0.0
+        Z = max(len(self.bw_bonds(a)) for a in range(self.b))
+        return Js.pop(), h, Z
+
+    def bw_bonds(self, a):
+        s = self.s
+        return [i for i, bd in enumerate(s.crystal.bonds) if bd.alpha == a and s.J.get(bd.shell, 0.0) != 0.0]
+
+    def emit_bitwise(self) -> None:
+        """Bitwise multi-spin Ising (SCK/bitwise.py:250-422). A lane owns one site of one word (32 replicas):
+        anti-aligned neighbour counts go into NP bit planes through a ripple adder, and the flip mask is
+        built from the table p[word, s, z_eff, k]. Shared mode: one uniform per (word, site), output w % 4 of
+        the Philox call with counter (w / 4) nsite + i. Independent mode: each replica's 32-bit U is compared
+        with P = floor(p 2^32) bit-sliced, most significant level first; four levels per Philox call with the
+        key of the level group, counter w nsite + i; a wave stops drawing once all its lanes are decided."""
+        w = self.w
+        s = self.s
+        J, h, Z = self.bitwise_params()
+        NP = max(1, Z.bit_length())
+        Z1 = Z + 1
+        NT = 2 * Z1 * Z1
+        field = h != 0.0
+        indep = self.mode == MODE_BITWISE_INDEP
+        static = (indep and s.acceptance != "barker" and s.mask is None and all(s.periodic))
+        nsite = self.NS
+        for a in range(self.b):
+            w(f"DEV uint32_t flip_{a}(const sck_args &A, const uint32_t *__restrict__ S, int wd, int own, int x, int y, "
+              f"int z, uint32_t key) {{")
+            w(f"    const int64_t wbase = (int64_t)wd * {nsite};")
+            w("    const uint32_t s0 = S[wbase + own];")
+            w("    int ze = 0;")
+            for p in range(NP):
+                w(f"    uint32_t c{p} = 0u;")
+            for i in self.bw_bonds(a):
+                q, ok = self.nb_index(i)
+                mk = " && A.mask[q]" if self.has_mask else ""
+                w(f"    {{ const int q = {q}; if (({ok}){mk}) {{ ze += 1; uint32_t cy = s0 ^ S[wbase + q];")
+                for p in range(NP):
+                    if p < NP - 1:
+                        w(f"        const uint32_t t{p} = c{p} & cy; c{p} ^= cy; cy = t{p};")
+                    else:
+                        w(f"        c{p} ^= cy;")
+                w("    } }")
+            eqs = []
+            for k in range(Z1):
+                terms = [f"c{p}" if (k >> p) & 1 else f"~c{p}" for p in range(NP)]
+                eqs.append(" & ".join(terms))
+            if not indep:
+                w(f"    const float *tb = (const float *)A.table + (int64_t)wd * {NT} + ze * {Z1};")
+                w(f"    uint32_t c0r = (uint32_t)(wd >> 2) * {nsite}u + (uint32_t)own, c1r = 0u, c2r = 0u, c3r = 0u;")
+                w("    philox10(c0r, c1r, c2r, c3r, key, 0u);")
+                w("    const int qsel = wd & 3;")
+                w("    const uint32_t raw = qsel == 0 ? c0r : (qsel == 1 ? c1r : (qsel == 2 ? c2r : c3r));")
+                w("    const float u = u01(raw);")
+                w("    uint32_t acc = 0u;")
+                for k in range(Z1):
+                    w(f"    {{ const uint32_t eq = {eqs[k]};")
+                    if field:
+                        w(f"      if (u < tb[{Z1 * Z1 + k}]) acc |= eq & s0;")
+                        w(f"      if (u < tb[{k}]) acc |= eq & ~s0; }}")
+                    else:
+                        w(f"      if (u < tb[{k}]) acc |= eq; }}")
+                w("    return acc;")
+                w("}")
+                w()
+                continue
+            # independent mode
+            w(f"    const int64_t *tb = (const int64_t *)A.table + (int64_t)wd * {NT} + ze * {Z1};")
+            w("    uint32_t acc = 0u, und = 0u;")
+            ent = []
+            for k in range(Z1):
+                sels = ([(f"(eq{k}) & ~s0", k, -1), (f"(eq{k}) & s0", Z1 * Z1 + k, 1)] if field else [(f"eq{k}", k, -1)])
+                w(f"    const uint32_t eq{k} = {eqs[k]};")
+                for sel, off, sigma in sels:
+                    if static and k <= Z and 2.0 * J * (Z - 2 * k) + 2.0 * h * sigma <= 0.0:
+                        w(f"    acc |= {sel};")
+                        continue
+                    e = len(ent)
+                    w(f"    const uint32_t e{e} = {sel};")
+                    w(f"    const int64_t Pq{e} = tb[{off}];")
+                    w(f"    if (Pq{e} >= 4294967296LL) acc |= e{e};")
+                    w(f"    if (Pq{e} > 0 && Pq{e} < 4294967296LL) und |= e{e};")
+                    w(f"    const uint32_t P{e} = (uint32_t)Pq{e};")
+                    ent.append(off)
+            if ent:
+                w("    const uint32_t ctr = (uint32_t)wd * " + f"{nsite}u + (uint32_t)own;")
+                w(f"    for (int g = 0; g < {LEVELS // 4}; ++g) {{")
+                w("        if (g >= 2 && __ballot(und != 0u) == 0ull) break;")
+                w(f"        const uint32_t kg = (uint32_t)(((int64_t)key + (int64_t)(g + 1) * {GROUP_KEY_STEP}LL) % 2147483629LL);")
+                w("        uint32_t r0 = ctr, r1 = 0u, r2 = 0u, r3 = 0u;")
+                w("        philox10(r0, r1, r2, r3, kg, 0u);")
+                w("        const uint32_t rw[4] = {r0, r1, r2, r3};")
+                w("#pragma unroll")
+                w("        for (int q = 0; q < 4; ++q) {")
+                w(f"            const uint32_t sh = {LEVELS - 1} - (4 * g + q);")
+                pb = " | ".join(f"(e{i} & (0u - ((P{i} >> sh) & 1u)))" for i in range(len(ent)))
+                w(f"            const uint32_t pb = {pb};")
+                w("            acc |= und & ~rw[q] & pb;")
+                w("            und &= ~(rw[q] ^ pb);")
+                w("        }")
+                w("    }")
+            w("    return acc;")
+            w("}")
+            w()
+        # class kernels: lane = site of the class row tile, blockIdx.y = word
+        for k, (c, a) in enumerate(self.classes):
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
+            w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
+            w("    const int wd = blockIdx.y;")
+            w(f"    const int row = t / {self.BX};")
+            w(f"    const int j = t - row * {self.BX};")
+            w(f"    const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+            w(f"    bool live = row < {self.ROWS};")
+            w(f"    int x0 = {self.P};")
+            for q in range(self.P - 1, -1, -1):
+                w(f"    if ({self.colour_expr(str(q), a)} == {c}) x0 = {q};")
+            w(f"    const int x = x0 + {self.P} * j;")
+            w(f"    live = live && x0 < {self.P} && x < {self.L1};")
+            w(f"    const int own = live ? {a * self.PL} + row * {self.L1} + x : 0;")
+            if self.has_mask:
+                w("    live = live && A.mask[own];")
+            if indep:
+                # every lane joins the wave-wide exit test of the level loop; dead lanes have nothing undecided
+                w("    uint32_t *S = (uint32_t *)A.spins;")
+                w("    const uint32_t key = sweep_key(A);")
+                w(f"    const uint32_t f = flip_{a}(A, S, wd, own, live ? x : 0, live ? y : 0, live ? z : 0, key);")
+                w(f"    if (live) S[(int64_t)wd * {nsite} + own] ^= f;")
+            else:
+                w("    if (!live) return;")
+                w("    uint32_t *S = (uint32_t *)A.spins;")
+                w("    const uint32_t key = sweep_key(A);")
+                w(f"    S[(int64_t)wd * {nsite} + own] ^= flip_{a}(A, S, wd, own, x, y, z, key);")
+            w("}")
+            w()
+        if indep:
+            # a dead lane must not decide anything: make its flip mask computation harmless
+            pass
+        w(f'extern "C" __global__ void __launch_bounds__(64) sck_seq(sck_args A) {{')
+        w("    const int wd = blockIdx.x * blockDim.x + threadIdx.x;")
+        w("    if (wd >= A.nrep) return;")
+        w("    const uint32_t key = sweep_key(A);")
+        w("    uint32_t *S = (uint32_t *)A.spins;")
+        for c, a in self.classes:
+            w(f"    for (int z = 0; z < {self.L3}; ++z) for (int y = 0; y < {self.L2}; ++y) for (int x = 0; x < {self.L1}; ++x) {{")
+            w(f"        if ({self.colour_expr('x', a)} != {c}) continue;")
+            w(f"        const int own = {a * self.PL} + (z * {self.L2} + y) * {self.L1} + x;")
+            if self.has_mask:
+                w("        if (!A.mask[own]) continue;")
+            w(f"        S[(int64_t)wd * {nsite} + own] ^= flip_{a}(A, S, wd, own, x, y, z, key);")
+            w("    }")
+        w("}")
+        w()
+        for a in range(self.b):
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_sync_{a}(sck_args A) {{')
+            w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
+            w("    const int wd = blockIdx.y;")
+            w(f"    bool live = t < {self.PL};")
+            w("    const int tt = live ? t : 0;")
+            w(f"    const int x = tt % {self.L1}, row = tt / {self.L1};")
+            w(f"    const int z = row / {self.L2}, y = row % {self.L2};")
+            w(f"    const int own = {a * self.PL} + tt;")
+            if self.has_mask:
+                w("    live = live && A.mask[own];")
+            w("    const uint32_t key = sweep_key(A);")
+            w(f"    const uint32_t f = flip_{a}(A, (const uint32_t *)A.spins, wd, own, x, y, z, key);")
+            w(f"    if (live) ((uint32_t *)A.dst)[(int64_t)wd * {nsite} + own] = ((const uint32_t *)A.spins)[(int64_t)wd * {nsite} + own] ^ f;")
+            w("}")
+            w()
+
+    # --------------------------------------------------------------------------------- observables
+    def emit_observe(self, chunk: int) -> None:
+        """Float64 energy and per-sublattice magnetization (SCK/sampler.py:296-345): each block sums the
+        sites [blockIdx.x * chunk, ...) of replica blockIdx.y into one partial; sck_obs_final adds the
+        partials of a replica in a fixed order."""
+        w, s, n, b = self.w, self.s, self.n, self.b
+        packed = self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP)
+        NV = 1 + b * n
+        w("DEV double spin_at(const sck_args &A, int rep, int comp, int q) {")
+        if packed:
+            w(f"    const uint32_t wv = ((const uint32_t *)A.spins)[(int64_t)(rep >> 5) * {self.NS} + q];")
+            w("    const double sv = ((wv >> (rep & 31)) & 1u) ? 1.0 : -1.0;")
+            w("    return A.mask[q] ? sv : 0.0;")
+        elif self.mode == MODE_INT8:
+            w(f"    return (double)((const int8_t *)A.spins)[(int64_t)rep * {n * self.NS} + (int64_t)comp * {self.NS} + q];")
+        else:
+            w(f"    return (double)((const float *)A.spins)[(int64_t)rep * {n * self.NS} + (int64_t)comp * {self.NS} + q];")
+        w("}")
+        w()
+        w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_obs(sck_args A) {{')
+        w("    const int rep = blockIdx.y;")
+        w(f"    double acc[{NV}];")
+        w(f"    for (int k = 0; k < {NV}; ++k) acc[k] = 0.0;")
+        w(f"    const int lo = blockIdx.x * {chunk};")
+        w(f"    const int hi = min(lo + {chunk}, {self.NS});")
+        w("    for (int site = lo + threadIdx.x; site < hi; site += blockDim.x) {")
+        w(f"        const int a = site / {self.PL}, rem = site - a * {self.PL};")
+        w(f"        const int x = rem % {self.L1}, row = rem / {self.L1};")
+        w(f"        const int z = row / {self.L2}, y = row % {self.L2};")
+        for k in range(n):
+            w(f"        const double s{k} = spin_at(A, rep, {k}, site);")
+        w("        double e = 0.0;")
+        for a in range(b):
+            w(f"        if (a == {a}) {{")
+            for k in range(n):
+                w(f"            acc[{1 + a * n + k}] += s{k};")
+            for i, bd in enumerate(s.crystal.bonds):
+                if bd.alpha != a:
+                    continue
+                M = s.bond_matrix(i)
+                if not M.any():
+                    continue
+                q, ok = self.nb_index(i)
+                w(f"            if ({ok}) {{ const int q = {q};")
+                for l in range(n):
+                    w(f"                const double n{l} = spin_at(A, rep, {l}, q);")
+                for k in range(n):
+                    terms = [f"{_f64(M[k, l])} * n{l}" for l in range(n) if M[k, l] != 0.0]
+                    if terms:
+                        w(f"                e -= 0.5 * s{k} * ({' + '.join(terms)});")
+                w("            }")
+            w("        }")
+        if s.h is not None:
+            for k in range(n):
+                if s.h[k] != 0.0:
+                    w(f"        e -= {_f64(s.h[k])} * s{k};")
+        if s.cubic_k1 is not None:
+            w(f"        e += (double)A.k1[site] * ((s0 * s1) * (s0 * s1) + (s0 * s2) * (s0 * s2) + (s1 * s2) * (s1 * s2))"
+              f" + {_f64(s.cubic_k2)} * (s0 * s1 * s2) * (s0 * s1 * s2);")
+        if s.uniaxial is not None:
+            Du, ax = s.uniaxial
+            se = " + ".join(f"s{k} * {_f64(ax[k])}" for k in range(n))
+            w(f"        {{ const double se = {se}; e -= {_f64(Du)} * se * se; }}")
+        if s.site_ku is not None:
+            sn = " + ".join(f"s{k} * (double)A.site_axis[(int64_t)site * {n} + {k}]" for k in range(n))
+            w(f"        {{ const double sn = {sn}; e -= (double)A.site_ku[site] * sn * sn; }}")
+        w("        acc[0] += e;")
+        w("    }")
+        # block reduction: wave butterfly, then LDS across the waves
+        w(f"    __shared__ double red[{BLOCK // 64}][{NV}];")
+        w(f"    for (int k = 0; k < {NV}; ++k) {{")
+        w("        double v = acc[k];")
+        w("        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);")
+        w("        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;")
+        w("    }")
+        w("    __syncthreads();")
+        w(f"    if (threadIdx.x < {NV}) {{")
+        w("        double v = 0.0;")
+        w(f"        for (int i = 0; i < {BLOCK // 64}; ++i) v += red[i][threadIdx.x];")
+        w(f"        A.obs_scratch[((int64_t)rep * gridDim.x + blockIdx.x) * {NV} + threadIdx.x] = v;")
+        w("    }")
+        w("}")
+        w()
+        w(f'extern "C" __global__ void __launch_bounds__(64) sck_obs_final(sck_args A) {{')
+        w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
+        w("    if (rep >= A.nrep) return;")
+        w(f"    const int nch = {self.obs_chunks};")
+        w(f"    for (int k = 0; k < {NV}; ++k) {{")
+        w("        double v = 0.0;")
+        w(f"        for (int c = 0; c < nch; ++c) v += A.obs_scratch[((int64_t)rep * nch + c) * {NV} + k];")
+        w("        if (k == 0) A.obs[rep] = v;")
+        w(f"        else A.obs[A.nrep + (int64_t)rep * {NV - 1} + (k - 1)] = v;")
+        w("    }")
+        w("}")
+        w()
+        w('extern "C" __global__ void sck_bump(sck_args A) {')
+        w("    if (threadIdx.x == 0 && blockIdx.x == 0) ((int64_t *)A.state)[1] += A.sweep_offset;")
+        w("}")
+
+    # ------------------------------------------------------------------------------------- driver
+    def generate(self) -> Module:
+        s = self.s
+        self.w(HEADER)
+        self.w("#pragma clang fp contract(off)")
+        self.w(f"// system: {s.name}; n={s.n} b={s.b} box={s.box} periodic={s.periodic}; "
+               f"colouring {s.colouring.describe()}; mode {self.mode}; proposal {self.proposal}")
+        self.w()
+        if self.mode in (MODE_INT8, MODE_F32):
+            self.emit_plain()
+        elif self.mode == MODE_PACKED:
+            self.emit_packed()
+        else:
+            self.emit_bitwise()
+        # observables: enough blocks per replica for a full chip when replicas are few
+        per = max(1, min(64, (4096 + self.B - 1) // max(1, self.B)))
+        chunk = max(BLOCK, -(-self.NS // per))
+        chunk = -(-chunk // BLOCK) * BLOCK
+        self.obs_chunks = -(-self.NS // chunk)
+        self.emit_observe(chunk)
+        src = "\n".join(self.lines) + "\n"
+        kernels = [f"sck_class_{k}" for k in range(len(self.classes))] + ["sck_seq"] + \
+                  [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"]
+        ki = {k: i for i, k in enumerate(kernels)}
+        R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
+        gy, gz = (R, 1) if R <= 65535 else (65535, -(-R // 65535))
+        sweep = []
+        for k in range(len(self.classes)):
+            if self.mode == MODE_PACKED:
+                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // (BLOCK // 32)), self.W, 1)))
+            else:
+                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // BLOCK), gy, gz)))
+        ref = [Launch(ki["sck_seq"], (-(-R // 64), 1, 1), (64, 1, 1))]
+        sync = []
+        for a in range(self.b):
+            if self.mode == MODE_PACKED:
+                sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.PL // (BLOCK // 32)), self.W, 1)))
+            else:
+                sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.PL // BLOCK), gy, gz)))
+        obs = [Launch(ki["sck_obs"], (self.obs_chunks, self.B, 1)),
+               Launch(ki["sck_obs_final"], (-(-self.B // 64), 1, 1), (64, 1, 1))]
+        bump = Launch(ki["sck_bump"], (1, 1, 1), (64, 1, 1))
+        layouts, offsets, total = ising_layout(s) if s.n == 1 else ([], [], 0)
+        mod = Module(src, kernels, sweep, ref, sync, obs, bump, self.obs_chunks, self.classes,
+                     table_entries=total, ising_layout=layouts)
+        mod.key = hashlib.sha256(src.encode()).hexdigest()[:16]
+        return mod
+
+
+def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None) -> Module:
+    """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
+    SCK/sampler.py:191-232 `_launch`)."""
+    if proposal not in ("uniform", "cone", "learned"):
+        raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
+    if proposal != "uniform" and sys_.n == 1:
+        raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
+    return _Gen(sys_, mode, batch, proposal, learned_meta).generate()

@@ -1,0 +1,455 @@
+"""Sampler: the reference's operator class over the MI355X kernels (SCK/sampler.py:39-345).
+
+Same constructor arguments, attributes and methods as the reference Sampler, with spins in the reference layout
+S[B, n, b, L3, L2, L1] (packed Ising: int32 words [W, 1, b, L3, L2, L1]). The sweep, the sequential chain, the
+synchronous control and the observables run as hipRTC-compiled kernels through libsck; there is no CPU path,
+and a Sampler on a non-HIP device raises.
+
+Additions: `proposal="cone"` (fixed half-angle per replica, `set_width`) and `proposal="learned"` (a width table
+from sck_amd.proposal) for XY/Heisenberg, and `observe(S)` returning energy and magnetization from one kernel.
+The Philox counter is 64-bit (identical to the reference's int32 counter below 2^31), so plain batches beyond
+the reference's limit run; the bitwise kernels keep the int32 counter w * nsite + i and its limit.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import hipgen as G
+from .system import System
+
+__all__ = ["Sampler", "COMPACT_MIN", "TILE", "MAX_BLOCK", "PTILE", "WORD", "BWG", "bitwise_check", "p_table",
+           "p_fixed"]
+
+COMPACT_MIN = 4
+TILE = 128
+MAX_BLOCK = 1024
+PTILE = 64
+WORD = 32
+BWG = 4
+
+
+# ------------------------------------------------------------------------------------- bitwise tables
+def bitwise_check(sys_: System) -> tuple[float, float, int]:
+    """(J, h, Z) of a system the bitwise kernel can run; ValueError otherwise (SCK/bitwise.py:59-71)."""
+    if sys_.n != 1:
+        raise ValueError("bitwise multi-spin coding is for Ising (n = 1) only")
+    if sys_.extra:
+        raise ValueError("bitwise multi-spin coding needs isotropic exchange only (no extra bond matrices)")
+    if sys_.cubic_k1 is not None:
+        raise ValueError("bitwise multi-spin coding does not support anisotropy")
+    Js = {float(sys_.J.get(bd.shell, 0.0)) for bd in sys_.crystal.bonds} - {0.0}
+    if len(Js) != 1:
+        raise ValueError(f"bitwise multi-spin coding needs one uniform coupling J on every bond, got {sorted(Js)}")
+    h = float(sys_.h[0]) if sys_.h is not None else 0.0
+    Z = max(sum(1 for bd in sys_.crystal.bonds if bd.alpha == a and sys_.J.get(bd.shell, 0.0) != 0.0)
+            for a in range(sys_.b))
+    return Js.pop(), h, Z
+
+
+def _p64(sys_: System, beta: np.ndarray) -> np.ndarray:
+    J, h, Z = bitwise_check(sys_)
+    beta = np.asarray(beta, np.float64).reshape(-1)
+    p = np.zeros((beta.size, 2, Z + 1, Z + 1), np.float64)
+    for si in (0, 1):
+        sigma = 2 * si - 1
+        for ze in range(Z + 1):
+            k = np.arange(ze + 1)
+            x = beta[:, None] * (2.0 * J * (ze - 2 * k) + 2.0 * h * sigma)[None, :]
+            if sys_.acceptance == "barker":
+                p[:, si, ze, :ze + 1] = 0.5 * (1.0 - np.tanh(0.5 * x))
+            else:
+                p[:, si, ze, :ze + 1] = np.exp(-np.maximum(x, 0.0))
+    return p
+
+
+def p_table(sys_: System, beta: np.ndarray) -> np.ndarray:
+    """Shared-mode acceptance p[word, s, z_eff, k], float32 (SCK/bitwise.py:79-84)."""
+    return _p64(sys_, beta).astype(np.float32)
+
+
+def p_fixed(sys_: System, beta: np.ndarray) -> np.ndarray:
+    """Independent mode P = floor(p 2^32) in [0, 2^32], int64 (SCK/bitwise.py:87-93)."""
+    return np.clip(np.floor(_p64(sys_, beta) * 2.0 ** 32).astype(np.int64), 0, 2 ** 32)
+
+
+def _stream_ptr(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class Sampler:
+    def __init__(self, system: System, batch: int, device="cuda", seed: int = 0, compact: bool = True,
+                 spin_dtype: torch.dtype | None = None, packed: bool | None = None, bitwise: bool = False,
+                 words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
+                 bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
+                 width=None, learned=None, cache_dir=None):
+        self.sys = system
+        self.bitwise = bool(bitwise)
+        self.independent = bool(independent)
+        if self.independent and not self.bitwise:
+            raise ValueError("independent per-replica draws need bitwise=True")
+        if self.bitwise:
+            _, _, self.zmax = bitwise_check(system)
+            if packed is False:
+                raise ValueError("bitwise multi-spin coding uses the packed storage")
+            packed = True
+        self.packed = (system.n == 1 and int(batch) >= WORD) if packed is None else bool(packed)
+        if self.packed and system.n != 1:
+            raise ValueError("bit packing is for Ising (n = 1) only")
+        self.spin_dtype = spin_dtype or (torch.int8 if system.n == 1 else torch.float32)
+        if not self.packed and self.spin_dtype not in (torch.int8, torch.float32):
+            raise ValueError("spins are stored as int8 (Ising) or float32")
+        if self.spin_dtype == torch.int8 and system.n != 1:
+            raise ValueError("int8 storage holds Ising spins only")
+        self.B = int(batch)
+        if self.B < 1:
+            raise ValueError("batch must be at least 1")
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("the MI355X backend runs on a HIP device (device='cuda'); it has no CPU path")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        n, b, L3, L2, L1 = system.n, system.b, system.L3, system.L2, system.L1
+        self.W = -(-self.B // WORD) if self.packed else self.B
+        self.Bp = self.W * WORD if self.packed else self.B
+        nsite = b * L3 * L2 * L1
+        if self.bitwise and self.W * nsite >= 2 ** 31 - 1:
+            raise ValueError("replica batch too large for the int32 Philox counter")
+        gmax = words_per_program or BWG
+        if gmax & (gmax - 1):
+            raise ValueError("words_per_program must be a power of two")
+        self.G = min(gmax, 1 << (self.W - 1).bit_length()) if self.bitwise else 1
+        self.ngroups = -(-self.W // self.G)
+        if max_block < 16 or max_block & (max_block - 1):
+            raise ValueError("max_block must be a power of two >= 16")
+        self.nrows = L3 * L2
+        P = G.x_period(system.colouring)
+        self.stride = P if (compact and P >= COMPACT_MIN) else 1
+        self.bx = -(-L1 // self.stride)
+        if proposal != "uniform" and n == 1:
+            raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
+        self.proposal = proposal
+        if self.bitwise:
+            self.mode = G.MODE_BITWISE_INDEP if self.independent else G.MODE_BITWISE
+        elif self.packed:
+            self.mode = G.MODE_PACKED
+        else:
+            self.mode = G.MODE_INT8 if self.spin_dtype == torch.int8 else G.MODE_F32
+        self.learned_meta = learned.meta() if learned is not None else None
+        self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta)
+        self.src = self.module.src
+        dev = self.device
+        m = system.site_mask()
+        self.mask_np = m
+        self.mask = torch.as_tensor(m.astype(np.int8), device=dev)
+        k1 = system.cubic_k1 if system.cubic_k1 is not None else np.zeros(m.shape, np.float32)
+        self.k1 = torch.as_tensor(np.asarray(k1, np.float32), device=dev)
+        self.state = torch.tensor([int(seed), 0], dtype=torch.int64, device=dev)
+        self.beta = torch.ones(self.Bp, dtype=torch.float32, device=dev)
+        self._beta_np = np.ones(self.Bp, np.float32)
+        if self.bitwise:
+            self._table = p_fixed if self.independent else p_table
+            self.ptab = torch.as_tensor(self._table(system, np.ones(self.W)), device=dev)
+            self.table = self.ptab
+        elif n == 1:
+            self.table = torch.as_tensor(G.ising_table(system, self._beta_np), device=dev).contiguous()
+        else:
+            self.table = torch.zeros(1, dtype=torch.float32, device=dev)
+        if proposal == "cone":
+            w = np.full(self.Bp, np.pi, np.float32) if width is None else np.broadcast_to(
+                np.asarray(width, np.float32).reshape(-1), (self.Bp,))
+            self.width = torch.as_tensor(np.ascontiguousarray(w), device=dev)
+        elif proposal == "learned":
+            self.width = torch.as_tensor(np.ascontiguousarray(learned.table(self._beta_np)), device=dev)
+            self._learned = learned
+        else:
+            self.width = torch.zeros(1, dtype=torch.float32, device=dev)
+        ku = system.site_ku if system.site_ku is not None else np.zeros(m.shape, np.float32)
+        ax = system.site_axis if system.site_axis is not None else np.zeros(m.shape + (n,), np.float32)
+        self.site_ku = torch.as_tensor(np.ascontiguousarray(ku, np.float32), device=dev)
+        self.site_axis = torch.as_tensor(np.ascontiguousarray(ax, np.float32), device=dev)
+        self.colours = system.colour_map()
+        self.classes = self.module.classes
+        self._mask_f = torch.as_tensor(m, device=dev).to(torch.float32)
+        nv = 1 + b * n
+        self._obs = torch.zeros(self.B * nv, dtype=torch.float64, device=dev)
+        self._obs_scratch = torch.zeros(self.B * self.module.obs_chunks * nv, dtype=torch.float64, device=dev)
+        self._handle = None
+        self._create(cache_dir)
+
+    # ------------------------------------------------------------------------------ native handle
+    def _create(self, cache_dir) -> None:
+        mod = self.module
+        names = (C.c_char_p * len(mod.kernels))(*[k.encode() for k in mod.kernels])
+        self._keep = [names]
+        d = L.Desc()
+        d.abi_version = L.SCK_ABI_VERSION
+        d.mode = self.mode
+        d.n, d.b, d.L1, d.L2, d.L3 = self.sys.n, self.sys.b, self.sys.L1, self.sys.L2, self.sys.L3
+        d.batch, d.words, d.obs_chunks = self.B, self.W, mod.obs_chunks
+        d.kernel_names = C.cast(names, C.POINTER(C.c_char_p))
+        d.n_kernels = len(mod.kernels)
+        for field_, plan in (("sweep", mod.sweep), ("ref", mod.ref), ("sync", mod.sync), ("obs", mod.obs)):
+            arr, cnt = L.launches(plan)
+            self._keep.append(arr)
+            setattr(d, field_, C.cast(arr, C.POINTER(L.LaunchC)))
+            setattr(d, "n_" + field_, cnt)
+        barr, _ = L.launches([mod.bump])
+        self._keep.append(barr)
+        d.bump = C.cast(barr, C.POINTER(L.LaunchC))
+        h = C.c_void_p()
+        cache = str(cache_dir or L.CACHE).encode()
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_create(C.byref(d), self.src.encode(), cache, C.byref(h)), "sck_create")
+        self._handle = h
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                L.lib().sck_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def _args(self, S: torch.Tensor, dst: torch.Tensor | None = None) -> L.Args:
+        a = L.Args()
+        a.spins = S.data_ptr()
+        a.dst = dst.data_ptr() if dst is not None else None
+        a.mask = self.mask.data_ptr()
+        a.k1 = self.k1.data_ptr()
+        a.beta = self.beta.data_ptr()
+        a.table = self.table.data_ptr()
+        a.state = self.state.data_ptr()
+        a.width = self.width.data_ptr()
+        a.site_ku = self.site_ku.data_ptr()
+        a.site_axis = self.site_axis.data_ptr()
+        a.obs = self._obs.data_ptr()
+        a.obs_scratch = self._obs_scratch.data_ptr()
+        a.sweep_offset = 0
+        a.nrep = self.W if self.mode in (G.MODE_PACKED, G.MODE_BITWISE, G.MODE_BITWISE_INDEP) else self.B
+        a.flags = 0
+        return a
+
+    def _check_spins(self, S: torch.Tensor) -> None:
+        s = self.sys
+        if self.packed:
+            shape, dt = (self.W, 1) + s.shape, torch.int32
+        else:
+            shape, dt = (self.B, s.n) + s.shape, self.spin_dtype
+        if tuple(S.shape) != shape or S.dtype != dt:
+            raise ValueError(f"spins must be {dt} of shape {shape}, got {S.dtype} {tuple(S.shape)}")
+        if S.device != self.device or not S.is_contiguous():
+            raise ValueError(f"spins must be a contiguous tensor on {self.device}")
+
+    # ------------------------------------------------------------------------------------ state
+    def set_temperature(self, T):
+        """beta = 1/T per replica, written in place (a captured graph reads the buffer by address)
+        (SCK/sampler.py:134-146)."""
+        T = np.asarray(torch.as_tensor(T, dtype=torch.float32).reshape(-1).cpu().numpy(), np.float32)
+        beta = (np.float32(1.0) / T).astype(np.float32)
+        beta = np.broadcast_to(beta, (self.B,)).copy() if beta.size == 1 else beta
+        if beta.size != self.B:
+            raise ValueError(f"need 1 or {self.B} temperatures, got {beta.size}")
+        if self.Bp > self.B:
+            beta = np.concatenate([beta, np.full(self.Bp - self.B, beta[-1], np.float32)])
+        if self.bitwise:
+            bwd = beta.reshape(self.W, WORD)
+            if not np.all(bwd == bwd[:, :1]):
+                raise ValueError("bitwise multi-spin coding needs one temperature per word of 32 replicas")
+            self.ptab.copy_(torch.as_tensor(self._table(self.sys, bwd[:, 0].astype(np.float64)), device=self.device))
+        elif self.sys.n == 1:
+            self.table.copy_(torch.as_tensor(G.ising_table(self.sys, beta), device=self.device))
+        if self.proposal == "learned":
+            self.width.copy_(torch.as_tensor(self._learned.table(beta), device=self.device))
+        self._beta_np = beta
+        self.beta.copy_(torch.as_tensor(beta, device=self.device))
+
+    def set_width(self, width) -> None:
+        """Cone half-angle per replica (proposal='cone'), in place."""
+        if self.proposal != "cone":
+            raise ValueError("set_width needs proposal='cone'")
+        w = np.broadcast_to(np.asarray(width, np.float32).reshape(-1), (self.Bp,)) if np.size(width) == 1 else \
+            np.asarray(width, np.float32).reshape(-1)
+        if w.size == self.B and self.Bp > self.B:
+            w = np.concatenate([w, np.full(self.Bp - self.B, w[-1], np.float32)])
+        if np.any(w <= 0) or np.any(w > np.pi + 1e-6):
+            raise ValueError("cone half-angles must lie in (0, pi]")
+        self.width.copy_(torch.as_tensor(np.ascontiguousarray(w), device=self.device))
+
+    def set_seed(self, seed: int, sweep: int = 0):
+        self.state[0] = int(seed)
+        self.state[1] = int(sweep)
+
+    def init_spins(self, kind="random", seed: int = 0) -> torch.Tensor:
+        """Initial spins drawn with the torch CPU generator as the reference draws them (SCK/sampler.py:152-167)."""
+        s = self.sys
+        g = torch.Generator(device="cpu").manual_seed(int(seed))
+        shape = (self.B, s.n) + s.shape
+        if kind == "ordered":
+            x = torch.zeros(shape)
+            x[:, 0] = 1.0
+        elif s.n == 1:
+            x = torch.randint(0, 2, shape, generator=g).float() * 2 - 1
+        else:
+            x = torch.randn(shape, generator=g)
+            x = x / x.norm(dim=1, keepdim=True)
+        x = x.to(self.device) * self._mask_f
+        if self.packed:
+            return self.pack(x)
+        return x.to(self.spin_dtype).contiguous()
+
+    def pack(self, x: torch.Tensor) -> torch.Tensor:
+        """(B, 1, b, L3, L2, L1) spins of +-1 -> (W, 1, b, L3, L2, L1) int32 words (SCK/sampler.py:169-178)."""
+        bits = (x > 0).to(torch.int64)
+        if self.Bp > self.B:
+            bits = torch.cat([bits, bits.new_zeros((self.Bp - self.B,) + tuple(bits.shape[1:]))])
+        bits = bits.reshape((self.W, WORD) + tuple(bits.shape[1:]))
+        sh = torch.arange(WORD, device=bits.device, dtype=torch.int64).reshape((1, WORD) + (1,) * (bits.dim() - 2))
+        word = (bits << sh).sum(dim=1)
+        word = torch.where(word >= 2 ** 31, word - 2 ** 32, word)
+        return word.to(torch.int32).contiguous()
+
+    def unpack(self, S: torch.Tensor) -> torch.Tensor:
+        """Packed words -> (B, 1, b, L3, L2, L1) int8 +-1, 0 on masked sites (SCK/sampler.py:180-188)."""
+        if not self.packed or S.dtype != torch.int32:
+            return S
+        w = S.to(torch.int64) & 0xFFFFFFFF
+        sh = torch.arange(WORD, device=S.device, dtype=torch.int64).reshape((1, WORD) + (1,) * (S.dim() - 1))
+        bits = (w.unsqueeze(1) >> sh) & 1
+        x = (bits * 2 - 1).reshape((self.Bp,) + tuple(S.shape[1:]))[: self.B]
+        return (x * self._mask_f.to(torch.int64)).to(torch.int8)
+
+    # ----------------------------------------------------------------------------------- sweeps
+    def sweep(self, S: torch.Tensor, n: int = 1, classes: tuple[int, int] | None = None) -> torch.Tensor:
+        """n coloured sweeps in place; advances the sweep counter (SCK/sampler.py:234-240). `classes=(lo, hi)`
+        restricts each sweep to class launches [lo, hi) (per-class checks)."""
+        self._check_spins(S)
+        lo, hi = classes if classes is not None else (0, -1)
+        a = self._args(S)
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_sweep(self._handle, C.byref(a), int(n), int(lo), int(hi),
+                                      C.c_void_p(_stream_ptr(self.device))), "sweep")
+        return S
+
+    def graph(self, S: torch.Tensor, k: int, observe: bool = False):
+        """Capture k sweeps (and, with observe=True, one observation after them) in a HIP graph; returns its
+        replay (SCK/sampler.py:242-269). The replay keeps the sampler and S alive."""
+        self._check_spins(S)
+        a = self._args(S)
+        g = C.c_void_p()
+        with torch.cuda.device(self.device):
+            torch.cuda.current_stream(self.device).synchronize()
+            L.check(L.lib().sck_graph_capture(self._handle, C.byref(a), int(k), int(bool(observe)), C.byref(g)),
+                    "graph capture")
+        dev = self.device
+        lib = L.lib()
+
+        def replay():
+            with torch.cuda.device(dev):
+                L.check(lib.sck_graph_replay(g, C.c_void_p(_stream_ptr(dev))), "graph replay")
+
+        def time_ms(reps: int) -> float:
+            ms = C.c_float()
+            with torch.cuda.device(dev):
+                L.check(lib.sck_time_replays(g, int(reps), C.c_void_p(_stream_ptr(dev)), C.byref(ms)), "timing")
+            return float(ms.value)
+
+        class _Keep:
+            def __init__(self, handle):
+                self.handle = handle
+
+            def __del__(self):
+                try:
+                    lib.sck_graph_destroy(self.handle)
+                except Exception:
+                    pass
+
+        replay.keep = (self, S, _Keep(g))
+        replay.time_ms = time_ms
+        return replay
+
+    def sync_sweep(self, S: torch.Tensor, n: int = 1) -> torch.Tensor:
+        """The synchronous control: every site updated from the old state (SCK/sampler.py:271-279)."""
+        self._check_spins(S)
+        dst = torch.empty_like(S)
+        a = self._args(S, dst)
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_sync_sweep(self._handle, C.byref(a), int(n), S.numel() * S.element_size(),
+                                           C.c_void_p(_stream_ptr(self.device))), "sync sweep")
+        return S
+
+    def reference_sweep(self, S: torch.Tensor, n: int = 1) -> torch.Tensor:
+        """Sequential reference chain: sites one at a time in class order, raster order inside a class, with
+        the same update and counters (SCK/sampler.py:281-293)."""
+        self._check_spins(S)
+        a = self._args(S)
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_reference_sweep(self._handle, C.byref(a), int(n),
+                                                C.c_void_p(_stream_ptr(self.device))), "reference sweep")
+        return S
+
+    # ------------------------------------------------------------------------------ observables
+    def observe(self, S: torch.Tensor):
+        """(E (B,) float64, M (B, b, n) float64) from the FP64 reduction kernels."""
+        self._check_spins(S)
+        a = self._args(S)
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_observe(self._handle, C.byref(a), C.c_void_p(_stream_ptr(self.device))), "observe")
+        B, b, n = self.B, self.sys.b, self.sys.n
+        return self._obs[:B].clone(), self._obs[B:].reshape(B, b, n).clone()
+
+    def energy(self, S: torch.Tensor) -> torch.Tensor:
+        """Total energy per replica, float64 (SCK/sampler.py:326-341)."""
+        if self.packed and S.dtype != torch.int32:
+            return self._energy_torch(S)
+        return self.observe(S)[0]
+
+    def magnetization(self, S: torch.Tensor) -> torch.Tensor:
+        """Per-sublattice magnetization (B, b, n), float64 (SCK/sampler.py:343-345)."""
+        if self.packed and S.dtype != torch.int32:
+            return S.double().sum(dim=(3, 4, 5)).transpose(1, 2)
+        return self.observe(S)[1]
+
+    def local_field(self, S: torch.Tensor) -> torch.Tensor:
+        """Bond part of the local field of every site by rolls, float64, as a reference (SCK/sampler.py:296-324)."""
+        s = self.sys
+        S64 = self.unpack(S).double()
+        v = torch.zeros_like(S64)
+        dims = {0: 5, 1: 4, 2: 3}
+        for i, bd in enumerate(s.crystal.bonds):
+            M = torch.as_tensor(s.bond_matrix(i), dtype=torch.float64, device=S.device)
+            if not M.any():
+                continue
+            src = S64[:, :, bd.alpha2]
+            valid = torch.ones(s.shape[1:], dtype=torch.float64, device=S.device)
+            for k in range(s.crystal.D):
+                d = bd.delta[k]
+                if d == 0:
+                    continue
+                src = torch.roll(src, shifts=-d, dims=dims[k] - 1)
+                if not s.periodic[k]:
+                    idx = torch.arange(s.box[k], device=S.device)
+                    ok = ((idx + d) >= 0) & ((idx + d) < s.box[k])
+                    shp = [1, 1, 1]
+                    shp[2 - k] = s.box[k]
+                    valid = valid * ok.to(torch.float64).reshape(shp)
+            src = src * valid
+            for k in range(s.n):
+                for l in range(s.n):
+                    if M[k, l] != 0:
+                        v[:, k, bd.alpha] += M[k, l] * src[:, l]
+        return v
+
+    def _energy_torch(self, x: torch.Tensor) -> torch.Tensor:
+        s = self.sys
+        S64 = x.double()
+        E = -0.5 * (S64 * self.local_field(x)).sum(dim=(1, 2, 3, 4, 5))
+        if s.h is not None:
+            E = E - (S64 * torch.as_tensor(s.h, dtype=torch.float64, device=x.device).reshape(1, -1, 1, 1, 1, 1)).sum(
+                dim=(1, 2, 3, 4, 5))
+        return E

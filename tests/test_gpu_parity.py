@@ -1,0 +1,264 @@
+"""Parity of the HIP sweep path with the reference's trajectories and the CPU oracle (needs a HIP device).
+
+Bar: Ising (int8, float32 storage, packed) and bitwise Ising bit-exact; XY/Heisenberg equal decisions except where
+the oracle's margin |beta dE + ln u| < 1e-5, spin values within 1e-5; observables within 1e-9 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from recipes import RECIPES
+from sck_amd import crystals as cr
+from sck_amd import system as sy
+from sck_amd.sampler import Sampler
+
+pytestmark = pytest.mark.gpu
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+DEV = "cuda"
+
+
+def _system(rc):
+    s = eval(rc["system"], {"sy": sy, "cr": cr, "np": np})
+    if rc.get("acceptance"):
+        s.acceptance = rc["acceptance"]
+    return s
+
+
+def _sampler_from_recipe(rc, s):
+    kw = dict(rc.get("sampler", {}))
+    A = Sampler(s, rc["B"], device=DEV, seed=rc["seed"], **kw)
+    A.set_temperature(torch.as_tensor(rc["T"], dtype=torch.float32))
+    return A
+
+
+# -------------------------------------------------------------------- against the reference's trajectories
+@pytest.mark.parametrize("name", list(RECIPES))
+def test_matches_reference_trajectory(name):
+    rc = RECIPES[name]
+    print(f"[gpu] reference trajectory {name}", flush=True)
+    z = np.load(GOLDEN / f"{name}.npz")
+    s = _system(rc)
+    A = _sampler_from_recipe(rc, s)
+    assert np.array_equal(A.beta.cpu().numpy(), z["beta"])
+    if A.bitwise:
+        assert np.array_equal(A.ptab.cpu().numpy(), z["ptab"])
+    S = torch.as_tensor(z["S0"]).to(DEV).contiguous()
+    assert S.dtype == A.init_spins(seed=rc["init_seed"]).dtype
+    assert torch.equal(S, A.init_spins(seed=rc["init_seed"]))         # the reference's initialisation
+    for k in range(rc["sweeps"]):
+        mode = rc.get("mode", "sweep")
+        if mode == "sync":
+            A.sync_sweep(S, 1)
+        elif mode == "reference":
+            A.reference_sweep(S, 1)
+        else:
+            A.sweep(S, 1)
+        got = S.cpu().numpy()
+        if s.n == 1:
+            assert np.array_equal(got, z["S"][k]), f"sweep {k}: {int((got != z['S'][k]).sum())} spins differ"
+        else:
+            assert np.allclose(got, z["S"][k], rtol=0, atol=1e-5), f"sweep {k}"
+    E, M = A.energy(S).cpu().numpy(), A.magnetization(S).cpu().numpy()
+    assert np.allclose(E, z["E"], rtol=1e-9, atol=1e-4)
+    assert np.allclose(M, z["M"], rtol=1e-9, atol=1e-4)
+
+
+# ----------------------------------------------------------------------------- parallel == sequential
+SMALL = [lambda: sy.ising("square", 4), lambda: sy.xy("square", 4), lambda: sy.heisenberg("bcc", 2),
+         lambda: sy.heisenberg("kagome", 3, J=-1.0), lambda: sy.nanodot(Rd=2.0, L=2),
+         lambda: sy.System("Ising sc odd periodic", 1, cr.make("sc"), (5, 5, 5), (True,) * 3, {1: 1.0}),
+         lambda: sy.System("Heisenberg sc 6x5x5 periodic", 3, cr.make("sc"), (6, 5, 5), (True,) * 3, {1: 1.0}),
+         lambda: sy.heisenberg("fcc", 2),
+         lambda: sy.System("XY sc 10x5x5 periodic", 2, cr.make("sc"), (10, 5, 5), (True,) * 3, {1: 1.0}),
+         lambda: sy.nanodot(Rd=2.5, L=3, shells=(1, 2, 4), J_shells={2: 0.3, 4: -0.1}),
+         lambda: sy.heisenberg("triangular", 24), lambda: sy.xy("square", 32)]
+
+
+@pytest.mark.parametrize("i", range(len(SMALL)))
+def test_parallel_equals_sequential_chain(i):
+    s = SMALL[i]()
+    print(f"[gpu] parallel == sequential {s.name} {s.box}", flush=True)
+    A, R = Sampler(s, 3, DEV, seed=11), Sampler(s, 3, DEV, seed=11)
+    for X in (A, R):
+        X.set_temperature(torch.tensor([0.7, 1.5, 3.0]))
+    S1 = A.init_spins(seed=2)
+    S2 = S1.clone()
+    for _ in range(2):
+        A.sweep(S1)
+        R.reference_sweep(S2)
+        assert torch.equal(S1, S2)
+    assert not torch.equal(S1, A.init_spins(seed=2))
+    assert int(A.state[1]) == 2 and int(R.state[1]) == 2
+
+
+def test_masked_sites_stay_zero_and_unit_norm_elsewhere():
+    s = sy.nanodot(Rd=2.0, L=2)
+    A = Sampler(s, 2, DEV, seed=3)
+    S = A.init_spins(seed=0)
+    A.sweep(S, 2)
+    m = torch.as_tensor(s.site_mask(), device=DEV)
+    norms = S.norm(dim=1)
+    assert torch.all(norms[:, ~m] == 0)
+    assert torch.allclose(norms[:, m], torch.ones_like(norms[:, m]), atol=1e-5)
+
+
+# ------------------------------------------------------------------------------ against the CPU oracle
+def test_ising_c1_shape_bit_exact_against_oracle():
+    """Ising square 64^2, int8, B = 512 (8 temperatures x 64 replicas in [1.5, 3.5]), seed 0: 30 sweeps on the GPU,
+    replicas 0, 1, 200, 447, 511 followed by the sequential oracle with the same counters."""
+    s = sy.ising("square", 64)
+    A = Sampler(s, 512, DEV, seed=0, packed=False)
+    T = np.repeat(np.linspace(1.5, 3.5, 8).astype(np.float32), 64)
+    A.set_temperature(torch.as_tensor(T))
+    S = A.init_spins(seed=0)
+    S0 = S.cpu().numpy()
+    A.sweep(S, 30)
+    reps = [0, 1, 200, 447, 511]
+    ref, _ = O.chain(O.OracleSystem(s), S0[reps].astype(np.float32), A.beta.cpu().numpy(), 0, 0, 30, reps=reps)
+    assert np.array_equal(S.cpu().numpy()[reps].astype(np.float32), ref)
+
+
+@pytest.mark.parametrize("acceptance", ["metropolis", "barker"])
+def test_packed_matches_int8_and_oracle(acceptance):
+    s = sy.System("Ising sc odd", 1, cr.make("sc"), (5, 5, 5), (True,) * 3, {1: 1.0}, h=(0.2,))
+    s.acceptance = acceptance
+    B = 40
+    P = Sampler(s, B, DEV, seed=5)
+    I8 = Sampler(s, B, DEV, seed=5, packed=False)
+    F = Sampler(s, B, DEV, seed=5, packed=False, spin_dtype=torch.float32)
+    assert P.packed and not I8.packed
+    T = torch.linspace(1.5, 4.0, B)
+    for X in (P, I8, F):
+        X.set_temperature(T)
+    Sp, S8, Sf = P.init_spins(seed=9), I8.init_spins(seed=9), F.init_spins(seed=9)
+    S0 = S8.cpu().numpy()
+    P.sweep(Sp, 3)
+    I8.sweep(S8, 3)
+    F.sweep(Sf, 3)
+    assert torch.equal(P.unpack(Sp), S8)
+    assert torch.equal(S8.float(), Sf)
+    ref, _ = O.chain(O.OracleSystem(s), S0.astype(np.float32), I8.beta.cpu().numpy(), 5, 0, 3)
+    assert np.array_equal(S8.cpu().numpy().astype(np.float32), ref)
+    assert torch.allclose(P.energy(Sp), I8.energy(S8))
+    Rp = Sampler(s, B, DEV, seed=5)
+    Rp.set_temperature(T)
+    Sr = Rp.init_spins(seed=9)
+    Rp.reference_sweep(Sr, 3)
+    assert torch.equal(Sr, Sp)
+
+
+@pytest.mark.parametrize("independent", [False, True])
+@pytest.mark.parametrize("acceptance", ["metropolis", "barker"])
+def test_bitwise_against_oracle(independent, acceptance):
+    s = sy.System("Ising sc 8 h", 1, cr.make("sc"), (8, 6, 4), (True, True, False), {1: 1.0}, h=(0.25,))
+    s.acceptance = acceptance
+    B = 32 * 6
+    A = Sampler(s, B, DEV, seed=21, bitwise=True, independent=independent)
+    A.set_temperature(torch.tensor([1.0, 2.0, 3.0, 4.0, 4.5, 6.0]).repeat_interleave(32))
+    S = A.init_spins(seed=4)
+    x = A.unpack(S).cpu().numpy()[:, 0]
+    for k in range(3):
+        A.sweep(S)
+        x = O.bitwise_chain(O.OracleSystem(s), x, A.ptab.cpu().numpy(), 21, k, 1, independent=independent)
+        assert np.array_equal(A.unpack(S).cpu().numpy()[:, 0], x), f"sweep {k}"
+    R = Sampler(s, B, DEV, seed=21, bitwise=True, independent=independent)
+    R.set_temperature(torch.tensor([1.0, 2.0, 3.0, 4.0, 4.5, 6.0]).repeat_interleave(32))
+    S2 = R.init_spins(seed=4)
+    R.reference_sweep(S2, 3)
+    assert torch.equal(S2, S)
+
+
+def test_bitwise_c2_shape_words_against_oracle():
+    """Bitwise independent, sc 128^3 periodic, B = 1024 (32 words), T = 4.5: two sweeps on the GPU; words are
+    independent chains, so the oracle follows words 0 and 31 over the whole lattice."""
+    s = sy.ising("sc", 128)
+    A = Sampler(s, 1024, DEV, seed=0, bitwise=True, independent=True)
+    A.set_temperature(4.5)
+    S = A.init_spins(seed=0)
+    S0 = S.cpu().numpy()
+    A.sweep(S, 2)
+    got = S.cpu().numpy()
+    os_ = O.OracleSystem(s)
+    for wd in (0, 31):
+        x = O.unpack(S0[wd:wd + 1], 32, s.site_mask())[:, 0]
+        tab = A.ptab.cpu().numpy()[wd:wd + 1]
+        x = O.bitwise_chain(os_, x, tab, 0, 0, 2, independent=True, word_offset=wd, threads=8)
+        assert np.array_equal(O.unpack(got[wd:wd + 1], 32, s.site_mask())[:, 0], x), f"word {wd}"
+
+
+VECTOR = [lambda: sy.heisenberg("bcc", 4), lambda: sy.xy("triangular", 12), lambda: sy.nanodot(Rd=3.0, L=3),
+          lambda: sy.System("Heis bcc J1J2 D h", 3, cr.make("bcc", (1, 2)), (4, 4, 4), (True,) * 3, {1: 1.0, 2: -0.2},
+                            h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0)))]
+
+
+@pytest.mark.parametrize("i", range(len(VECTOR)))
+def test_vector_decisions_match_oracle_within_tolerance(i):
+    """Per class launch from the same state: the GPU's accept decisions equal the oracle's except where the
+    oracle's margin |beta dE + ln u| < 1e-5; accepted spins agree to 1e-5."""
+    s = VECTOR[i]()
+    B = 16
+    A = Sampler(s, B, DEV, seed=7)
+    A.set_temperature(torch.linspace(0.4, 3.0, B))
+    S = A.init_spins(seed=5)
+    os_ = O.OracleSystem(s)
+    beta = A.beta.cpu().numpy()
+    flips = 0
+    for sweep in range(2):
+        for c in range(len(A.classes)):
+            x0 = S.cpu().numpy()
+            A.state[1] = sweep
+            A.sweep(S, 1, classes=(c, c + 1))
+            ref, _, mg = O.chain(os_, x0, beta, 7, sweep, 1, class_range=(c, c + 1), margins=True)
+            got = S.cpu().numpy()
+            bad = np.abs(got - ref).max(axis=1) > 1e-5              # (B, b, L3, L2, L1)
+            near = np.abs(mg.reshape(bad.shape)) < 1e-5
+            assert not np.any(bad & ~near), f"sweep {sweep} class {c}: {int((bad & ~near).sum())} decisions differ"
+            flips += int(bad.sum())
+            S.copy_(torch.as_tensor(ref, device=DEV))               # continue from the oracle's state
+    assert flips <= 2
+
+
+def test_observables_match_oracle():
+    for s in (sy.nanodot(Rd=3.0, L=3), sy.heisenberg("kagome", 6, J=-1.0), sy.ising("triangular", 6),
+              sy.System("u", 3, cr.make("bcc", (1, 2)), (4, 4, 4), (True,) * 3, {1: 1.0, 2: -0.2},
+                        h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0)))):
+        A = Sampler(s, 5, DEV, seed=1, packed=False)
+        S = A.init_spins(seed=3)
+        E, M = A.observe(S)
+        Eo, Mo = O.observe(O.OracleSystem(s), S.cpu().numpy())
+        assert np.allclose(E.cpu().numpy(), Eo, rtol=1e-12, atol=1e-9)
+        assert np.allclose(M.cpu().numpy(), Mo, rtol=1e-12, atol=1e-9)
+
+
+def test_graph_replay_equals_eager_sweeps_and_reads_beta_by_address():
+    s = sy.heisenberg("bcc", 4)
+    A, Bs = Sampler(s, 8, DEV, seed=2), Sampler(s, 8, DEV, seed=2)
+    for X in (A, Bs):
+        X.set_temperature(1.2)
+    S1 = A.init_spins(seed=1)
+    S2 = S1.clone()
+    rep = A.graph(S1, 3)
+    rep()
+    Bs.sweep(S2, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(S1, S2) and int(A.state[1]) == 3
+    A.set_temperature(0.5)                                     # in place: the graph sees it
+    Bs.set_temperature(0.5)
+    for _ in range(3):
+        rep()
+        Bs.sweep(S2, 3)
+        torch.cuda.synchronize()
+        assert torch.equal(S1, S2)
+    assert int(A.state[1]) == 12
+
+
+def test_sync_sweep_matches_oracle():
+    s = sy.xy("square", 8)
+    A = Sampler(s, 4, DEV, seed=3)
+    A.set_temperature(0.9)
+    S = A.init_spins(seed=1)
+    x = O.sync(O.OracleSystem(s), S.cpu().numpy(), A.beta.cpu().numpy(), 3, 0, 1)
+    A.sync_sweep(S, 1)
+    assert np.allclose(S.cpu().numpy(), x, atol=1e-5)

@@ -1,0 +1,142 @@
+"""Host-side system model, generator and C-ABI library (CPU only: nothing here launches a kernel).
+
+Known answers follow the reference's tests (tests/test_core.py:12-36, 118-130; tests/test_bitwise.py:119-147).
+"""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from sck_amd import colouring as co
+from sck_amd import crystals as cr
+from sck_amd import hipgen as hg
+from sck_amd import system as sy
+
+REPO = Path(__file__).resolve().parents[1]
+
+EXPECTED = {("square", (1,)): (2, 2), ("sc", (1,)): (2, 2), ("sc", (1, 2)): (4, 4), ("sc", (1, 2, 3, 4)): (9, 8),
+            ("bcc", (1,)): (2, 2), ("fcc", (1,)): (4, 4), ("triangular", (1,)): (3, 3),
+            ("honeycomb", (1,)): (2, 2), ("kagome", (1,)): (3, 3)}
+
+
+@pytest.mark.parametrize("key", sorted(EXPECTED))
+def test_colouring_counts_and_admissibility(key):
+    c = cr.make(*key)
+    col = co.search(c)
+    assert co.admissible(col, c)
+    assert (col.n_colours, co.clique_number(c)) == EXPECTED[key]
+
+
+def test_bond_counts():
+    assert [len(cr.make(n).bonds) for n in ("square", "sc", "bcc", "fcc", "triangular", "kagome")] == [4, 6, 16, 48, 6, 12]
+
+
+def test_seam_condition_rejects_odd_box():
+    c = cr.make("triangular")
+    col = co.search(c)
+    assert not co.seam_ok(col, (4, 4), (True, True))
+    with pytest.raises(ValueError):
+        sy.System("bad", 1, c, (4, 4), (True, True), {1: 1.0}, colouring=col)
+
+
+def test_x_period_of_odd_periodic_box():
+    s = sy.System("Ising sc odd periodic", 1, cr.make("sc"), (5, 5, 5), (True,) * 3, {1: 1.0})
+    assert s.colouring.n_colours == 5 and hg.x_period(s.colouring) == 5
+
+
+def test_large_boxes_colour_quickly_and_match_the_survey():
+    """Triangular 2048^2 cannot take Z3 (2048 mod 3 != 0): four colours Z2 x Z2; bcc shells 1-2: four classes."""
+    t = sy.xy("triangular", 2048)
+    assert t.colouring.m == (2, 2) and len(t.classes()) == 4
+    b = sy.System("bcc J1J2", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3, {1: 1.0, 2: -0.2})
+    assert b.colouring.n_colours == 4 and len(b.classes()) == 4
+    s = sy.ising("sc", 128)
+    assert len(s.classes()) == 2
+
+
+def test_vectorised_colour_map_equals_the_definition():
+    s = sy.nanodot(Rd=3.0, L=3)
+    cm = s.colour_map()
+    for a in range(s.b):
+        for z in range(s.L3):
+            for y in range(s.L2):
+                for x in range(s.L1):
+                    assert cm[a, z, y, x] == s.colouring.colour((x, y, z), a)
+
+
+def test_nanodot_defaults_and_shells():
+    s = sy.nanodot(Rd=2.5, L=3)
+    assert s.J == {1: 1.0} and s.colouring.n_colours == 2 and len(s.extra) == 4
+    s2 = sy.nanodot(Rd=2.5, L=3, shells=(1, 2, 4), J_shells={2: 0.3, 4: -0.1})
+    assert s2.J == {1: 1.0, 2: 0.3, 4: -0.1} and s2.colouring.n_colours > 2
+
+
+def test_ising_table_dE_values_square():
+    """Square lattice, J = 1, h = 0: dE = 2 s sum_j s_j over sums -4..4 (indices of the table digits)."""
+    dE = hg.ising_dE_values(sy.ising("square", 4))
+    g = np.repeat(np.arange(-4, 5), 2)
+    s = np.tile([-1, 1], 9)
+    assert np.array_equal(dE, (2 * s * g).astype(np.float32))
+
+
+def test_ising_table_two_groups_in_reference_order():
+    s = sy.System("J1-J2", 1, cr.make("sc", (1, 2)), (4, 4, 4), (True,) * 3, {1: 1.0, 2: 0.3}, h=(0.1,))
+    layouts, offsets, total = hg.ising_layout(s)
+    assert [d[1] for d in layouts[0]] == [13, 25, 2] and total == 13 * 25 * 2
+    dE = hg.ising_dE_values(s)
+    # entry (g1=6, g2=12, s=+1): v = (0 + 1*6) + fp32(0.3)*12 + fp32(0.1), dE = 2 v in fp32
+    v = np.float32(np.float32(0) + np.float32(1.0) * np.float32(6))
+    v = np.float32(v + np.float32(0.3) * np.float32(12))
+    v = np.float32(v + np.float32(0.1))
+    assert dE[(12 * 25 + 24) * 2 + 1] == np.float32(2) * v
+
+
+def test_metropolis_table_uses_float32_exp_of_float32_product():
+    s = sy.ising("square", 4)
+    beta = np.float32(1) / np.array([2.0, 3.0], np.float32)
+    t = hg.ising_table(s, beta)
+    dE = hg.ising_dE_values(s)
+    assert np.array_equal(t, np.exp(((-beta)[:, None] * dE[None, :]).astype(np.float32)))
+
+
+def test_generator_modes_emit_all_kernels():
+    for s, mode in [(sy.ising("square", 8), hg.MODE_INT8), (sy.xy("square", 4), hg.MODE_F32),
+                    (sy.nanodot(Rd=2.0, L=2), hg.MODE_F32), (sy.ising("square", 4), hg.MODE_PACKED),
+                    (sy.ising("sc", 4), hg.MODE_BITWISE), (sy.ising("sc", 4), hg.MODE_BITWISE_INDEP)]:
+        m = hg.generate(s, mode, 64)
+        for k in m.kernels:
+            assert re.search(rf"__global__ void .*\b{k}\(sck_args A\)", m.src), k
+        assert len(m.sweep) == len(s.classes())
+
+
+def test_library_exports_every_declared_symbol():
+    from sck_amd import _lib as L
+    header = (REPO / "include" / "sck.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(sck_\w+)\(", header, re.M))
+    assert declared == set(L.EXPORTS)
+    lib = ctypes.CDLL(str(L.LIB))
+    for name in declared:
+        assert hasattr(lib, name)
+    assert L.lib().sck_version() == 1
+
+
+def test_hiprtc_compiles_generated_modules_without_a_device(tmp_path):
+    from sck_amd import _lib as L
+    for s, mode in [(sy.heisenberg("bcc", 2), hg.MODE_F32), (sy.ising("sc", 4), hg.MODE_BITWISE_INDEP)]:
+        L.precompile(hg.generate(s, mode, 64).src, tmp_path)
+    assert len(list(tmp_path.glob("*.co"))) == 2
+
+
+def test_bad_source_reports_the_compiler_log(tmp_path):
+    from sck_amd import _lib as L
+    with pytest.raises(RuntimeError, match="hipRTC"):
+        L.precompile('extern "C" __global__ void k() { this is not code }', tmp_path)
+
+
+def test_sampler_refuses_cpu_device():
+    from sck_amd.sampler import Sampler
+    with pytest.raises(ValueError, match="HIP device"):
+        Sampler(sy.ising("square", 4), 2, device="cpu")
