@@ -57,7 +57,7 @@ typedef struct sck_args {
     double *obs_scratch;      /* [B][chunks][1 + b*n] per-block partial sums                      */
     int64_t sweep_offset;     /* sweep index = state[1] + sweep_offset                            */
     int32_t nrep;             /* replicas B (words W for packed/bitwise)                          */
-    int32_t flags;            /* reserved, 0                                                      */
+    int32_t nsweep;           /* sweeps one launch of a resident kernel performs (set by the library) */
 } sck_args;
 
 /* One kernel launch of a plan. */
@@ -82,6 +82,8 @@ typedef struct sck_desc {
     const sck_launch *sync;   int32_t n_sync;    /* one synchronous-control sweep                    */
     const sck_launch *obs;    int32_t n_obs;     /* energy and magnetization                          */
     const sck_launch *bump;                      /* state[1] += sweep_offset                          */
+    const sck_launch *resident;                  /* NULL, or whole sweeps with each replica's lattice in LDS:
+                                                    one launch performs args.nsweep sweeps                */
 } sck_desc;
 
 typedef struct sck_handle sck_handle;

@@ -139,6 +139,8 @@ struct sck_handle {
     std::vector<std::string> names;
     std::vector<sck_launch> sweep, ref, sync, obs;
     sck_launch bump{};
+    bool has_resident = false;
+    sck_launch resident{};
     int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
 };
 
@@ -170,6 +172,14 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
     if (hi < 0 || hi > ns) hi = ns;
     if (lo < 0 || lo > hi) return fail(SCK_E_INVALID, "class range out of bounds");
     sck_args a = *args;
+    if (h->has_resident && lo == 0 && hi == ns && n > 0) {
+        // whole sweeps: one launch, each workgroup runs its replica's n sweeps with the lattice in LDS
+        a.sweep_offset = 0;
+        a.nsweep = n;
+        int rc = launch(h, h->resident, a, s);
+        if (rc) return rc;
+        return bump(h, *args, n, s);
+    }
     for (int32_t i = 0; i < n; ++i) {
         a.sweep_offset = i;
         for (int k = lo; k < hi; ++k) {
@@ -231,6 +241,10 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
     h->sync.assign(d->sync, d->sync + d->n_sync);
     h->obs.assign(d->obs, d->obs + d->n_obs);
     h->bump = *d->bump;
+    if (d->resident) {
+        h->has_resident = true;
+        h->resident = *d->resident;
+    }
     h->mode = d->mode;
     h->n = d->n;
     h->b = d->b;

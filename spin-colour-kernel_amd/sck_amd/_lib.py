@@ -30,7 +30,7 @@ class Args(C.Structure):
                 ("beta", C.c_void_p), ("table", C.c_void_p), ("state", C.c_void_p), ("width", C.c_void_p),
                 ("site_ku", C.c_void_p), ("site_axis", C.c_void_p), ("obs", C.c_void_p),
                 ("obs_scratch", C.c_void_p), ("sweep_offset", C.c_int64), ("nrep", C.c_int32),
-                ("flags", C.c_int32)]
+                ("nsweep", C.c_int32)]
 
 
 class LaunchC(C.Structure):
@@ -46,7 +46,7 @@ class Desc(C.Structure):
                 ("ref", C.POINTER(LaunchC)), ("n_ref", C.c_int32),
                 ("sync", C.POINTER(LaunchC)), ("n_sync", C.c_int32),
                 ("obs", C.POINTER(LaunchC)), ("n_obs", C.c_int32),
-                ("bump", C.POINTER(LaunchC))]
+                ("bump", C.POINTER(LaunchC)), ("resident", C.POINTER(LaunchC))]
 
 
 #: every symbol include/sck.h declares

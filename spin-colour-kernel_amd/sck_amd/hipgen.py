@@ -74,6 +74,7 @@ class Module:
     table_entries: int = 0            # Ising: table entries per replica
     ising_layout: list = field(default_factory=list)
     key: str = ""
+    resident: Launch | None = None     # whole sweeps of one replica per workgroup with the lattice in LDS
 
 
 def _hexf(v: float) -> str:
@@ -199,7 +200,7 @@ HEADER = r"""
 typedef struct sck_args {
     void *spins; void *dst; const int8_t *mask; const float *k1; const float *beta; const void *table;
     const int64_t *state; const float *width; const float *site_ku; const float *site_axis;
-    double *obs; double *obs_scratch; int64_t sweep_offset; int32_t nrep; int32_t flags;
+    double *obs; double *obs_scratch; int64_t sweep_offset; int32_t nrep; int32_t nsweep;
 } sck_args;
 
 #define DEV static __device__ __forceinline__
@@ -304,6 +305,20 @@ class _Gen:
             expr = term if expr is None else f"(({expr}) * {mr} + {term})"
         return expr
 
+    # ------------------------------------------------------------------------ Ising table decisions
+    def decided_expr(self, p: str) -> str:
+        """True where the table value alone decides the move (no draw needed): Metropolis p >= 1 (every
+        u0 <= 1 - 2^-24 is below it) or p == 0 (u0 >= 0 never is); Barker q == 1 (u0 * 1 < 1) or q == inf."""
+        if self.s.acceptance == "barker":
+            return f"({p} == 1.0f || !({p} < __builtin_inff()))"
+        return f"({p} >= 1.0f || !({p} > 0.0f))"
+
+    def decided_value(self, p: str) -> str:
+        return f"({p} == 1.0f ? 1 : 0)" if self.s.acceptance == "barker" else f"({p} >= 1.0f ? 1 : 0)"
+
+    def accept_expr(self, p: str) -> str:
+        return f"((u0 * {p}) < 1.0f)" if self.s.acceptance == "barker" else f"(u0 < {p})"
+
     # ------------------------------------------------------------------------------- site update
     def emit_update_plain(self, a: int) -> None:
         """__device__ update of one site of sublattice a for one replica (unpacked storage).
@@ -311,16 +326,21 @@ class _Gen:
         s, n = self.s, self.n
         T = "int8_t" if self.mode == MODE_INT8 else "float"
         w = self.w
-        w(f"DEV int upd_{a}(const sck_args &A, const {T} *__restrict__ src, {T} *__restrict__ dst, int64_t rbase, "
-          f"int rep, int own, int x, int y, int z, uint32_t key) {{")
         groups, extras = _groups(s, a)
         if n == 1:
             layouts, offsets, total = ising_layout(s)
             digits = layouts[a]
-            w("    int idx = 0;")
+            # table lookup: p (Metropolis) or 1 + exp(beta dE) (Barker) of the site's neighbour configuration
+            w(f"DEV float isp_{a}(const {T} *__restrict__ src, int64_t rbase, int own, int x, int y, int z, "
+              f"const float *__restrict__ tb, int *s_out) {{")
             w(f"    const int s0 = (int)src[rbase + own];")
+            w("    *s_out = s0;")
+            ng = sum(1 for Jv, _ in groups if Jv != 0.0)
+            for k in range(ng):
+                w(f"    int gsum{k} = 0;")
+            for k in range(len(extras)):
+                w(f"    int ext{k} = 0;")
             gi = 0
-            slot = {}
             for Jv, members in groups:
                 w("    {")
                 w("        int g = 0;")
@@ -330,20 +350,9 @@ class _Gen:
                       + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
                 if Jv != 0.0:
                     w(f"        gsum{gi} = g + {len(members)};")
-                    slot[gi] = True
                     gi += 1
                 w("    }")
-            # declare before use: rewrite with declarations up front
-            decl = [f"    int gsum{k} = 0;" for k in range(gi)] + [f"    int ext{k} = 0;" for k in range(len(extras))]
-            # insert declarations after the s0 line
-            pos = len(self.lines) - 1
-            while not self.lines[pos].startswith("    const int s0"):
-                pos -= 1
-            self.lines[pos + 1:pos + 1] = decl
-            # index: digits in order, last fastest
-            terms = []
-            gk = 0
-            ek = 0
+            terms, gk, ek = [], 0, 0
             for kind, size, payload in digits:
                 if kind == "g":
                     terms.append((f"gsum{gk}", size))
@@ -356,29 +365,25 @@ class _Gen:
             expr = "0"
             for t, size in terms:
                 expr = f"({expr}) * {size} + {t}"
-            w(f"    idx = {offsets[a]} + {expr};")
-            w(f"    const float p = ((const float *)A.table)[(int64_t)rep * {total} + idx];")
-            barker = s.acceptance == "barker"
-            if barker:
-                w("    int acc;")
-                w("    if (p == 1.0f) acc = 1;")               # u0 * 1 < 1 for every u0 < 1
-                w("    else if (!(p < __builtin_inff())) acc = 0;")   # u0 * inf is inf or NaN: never below 1
-                w("    else {")
-            else:
-                w("    int acc;")
-                w("    if (p >= 1.0f) acc = 1;")               # every u0 <= 1 - 2^-24 is below p
-                w("    else if (!(p > 0.0f)) acc = 0;")        # u0 >= 0 is never below 0
-                w("    else {")
+            w(f"    return tb[{offsets[a]} + {expr}];")
+            w("}")
+            w()
+            w(f"DEV int upd_{a}(const sck_args &A, const {T} *__restrict__ src, {T} *__restrict__ dst, int64_t rbase, "
+              f"int rep, int own, int x, int y, int z, uint32_t key) {{")
+            w("    int s0;")
+            w(f"    const float p = isp_{a}(src, rbase, own, x, y, z, (const float *)A.table + (int64_t)rep * {total}, &s0);")
+            w("    int acc;")
+            w(f"    if ({self.decided_expr('p')}) acc = {self.decided_value('p')};")
+            w("    else {")
             self.emit_draw("        ")
-            if barker:
-                w("        acc = (u0 * p) < 1.0f;")
-            else:
-                w("        acc = u0 < p;")
+            w(f"        acc = {self.accept_expr('p')};")
             w("    }")
             w(f"    if (acc) dst[rbase + own] = ({T})(-s0);")
             w("    return acc;")
             w("}")
             return
+        w(f"DEV int upd_{a}(const sck_args &A, const {T} *__restrict__ src, {T} *__restrict__ dst, int64_t rbase, "
+          f"int rep, int own, int x, int y, int z, uint32_t key) {{")
         # vector models: fp32 local field in the reference's order
         for k in range(n):
             w(f"    float s{k} = (float)src[rbase + {k * self.NS} + own];")
@@ -592,6 +597,115 @@ class _Gen:
         for q in range(self.P - 1, -1, -1):
             w(f"    if ({self.colour_expr(str(q), a)} == {c}) x0 = {q};")
         w(f"    if (x0 >= {self.P}) return;")
+
+    # ---------------------------------------------------------------------------- resident kernel
+    RESIDENT_THREADS = 256
+    RESIDENT_LDS_MAX = 60 * 1024
+
+    def resident_plan(self):
+        """(chunk, LDS bytes) of the LDS-resident kernel, or None when one replica's lattice does not fit."""
+        if self.mode not in (MODE_INT8, MODE_F32):
+            return None
+        elt = 1 if self.mode == MODE_INT8 else 4
+        lat = self.n * self.NS * elt
+        lat = -(-lat // 16) * 16
+        RT = self.RESIDENT_THREADS
+        slots = max(self.ROWS * self.BX, 1)
+        chunk = min(-(-slots // RT) * RT, 16 * RT)
+        lds = lat
+        if self.n == 1:
+            _, _, total = ising_layout(self.s)
+            lds += 4 * total + 8 * chunk + 16
+        return (chunk, lds) if lds <= self.RESIDENT_LDS_MAX else None
+
+    def emit_resident(self, plan) -> None:
+        """sck_resident: one workgroup per replica keeps that replica's lattice in LDS for A.nsweep whole
+        sweeps (barriers between classes replace kernel boundaries; replicas are independent chains, so no
+        workgroup waits for another). Ising: each class pass first decides every site whose table value alone
+        decides it and queues the others with their p in LDS, then draws Philox only for the queue, densely."""
+        w = self.w
+        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        RT = self.RESIDENT_THREADS
+        chunk, lds = plan
+        NSn = self.n * self.NS
+        w(f'extern "C" __global__ void __launch_bounds__({RT}) sck_resident(sck_args A) {{')
+        w(f"    __shared__ {T} lat[{NSn}];")
+        if self.n == 1:
+            _, _, total = ising_layout(self.s)
+            w(f"    __shared__ float tbl[{total}];")
+            w(f"    __shared__ int q_own[{chunk}];")
+            w(f"    __shared__ float q_p[{chunk}];")
+            w("    __shared__ int q_n;")
+        w("    const int rep = blockIdx.x;")
+        w("    if (rep >= A.nrep) return;")
+        w(f"    {T} *g = ({T} *)A.spins + (int64_t)rep * {NSn};")
+        w(f"    for (int i = threadIdx.x; i < {NSn}; i += {RT}) lat[i] = g[i];")
+        if self.n == 1:
+            w(f"    for (int i = threadIdx.x; i < {total}; i += {RT}) tbl[i] = ((const float *)A.table)[(int64_t)rep * {total} + i];")
+        w("    __syncthreads();")
+        w("    const int lane = threadIdx.x & 63;")
+        w("    for (int it = 0; it < A.nsweep; ++it) {")
+        w("        sck_args Ai = A;")
+        w("        Ai.sweep_offset = A.sweep_offset + it;")
+        w("        const uint32_t key = sweep_key(Ai);")
+        slots = self.ROWS * self.BX
+        for k, (c, a) in enumerate(self.classes):
+            w(f"        // class {k}: colour {c}, sublattice {a}")
+            if self.n == 1:
+                w(f"        for (int base = 0; base < {slots}; base += {chunk}) {{")
+                w("            if (threadIdx.x == 0) q_n = 0;")
+                w("            __syncthreads();")
+                w(f"            for (int t = base + threadIdx.x; t < base + {chunk}; t += {RT}) {{")
+            else:
+                w(f"        for (int t = threadIdx.x; t < {-(-slots // RT) * RT}; t += {RT}) {{")
+                w("            {")
+            w(f"                const int row = t / {self.BX};")
+            w(f"                const int j = t - row * {self.BX};")
+            w(f"                const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+            w(f"                int x0 = {self.P};")
+            for q in range(self.P - 1, -1, -1):
+                w(f"                if ({self.colour_expr(str(q), a)} == {c}) x0 = {q};")
+            w(f"                const int x = x0 + {self.P} * j;")
+            w(f"                bool valid = t < {slots} && row < {self.ROWS} && x0 < {self.P} && x < {self.L1};")
+            w(f"                const int own = valid ? {a * self.PL} + row * {self.L1} + x : 0;")
+            if self.has_mask:
+                w("                valid = valid && A.mask[own];")
+            if self.n == 1:
+                w("                int s0 = 1;")
+                w(f"                const float p = valid ? isp_{a}(lat, 0, own, x, y, z, tbl, &s0) : 0.0f;")
+                w(f"                const bool need = valid && !{self.decided_expr('p')};")
+                w(f"                if (valid && !need && {self.decided_value('p')}) lat[own] = ({T})(-s0);")
+                w("                const unsigned long long m = __ballot(need);")
+                w("                if (m) {")
+                w("                    int qb = 0;")
+                w("                    if (lane == 0) qb = atomicAdd(&q_n, __popcll(m));")
+                w("                    qb = __shfl(qb, 0, 64);")
+                w("                    if (need) {")
+                w("                        const int pos = qb + __popcll(m & ((1ull << lane) - 1ull));")
+                w("                        q_own[pos] = own;")
+                w("                        q_p[pos] = p;")
+                w("                    }")
+                w("                }")
+                w("            }")
+                w("            __syncthreads();")
+                w("            const int nq = q_n;")
+                w(f"            for (int e = threadIdx.x; e < nq; e += {RT}) {{")
+                w("                const int own = q_own[e];")
+                w("                const float p = q_p[e];")
+                self.emit_draw("                ")
+                w(f"                if ({self.accept_expr('p')}) lat[own] = ({T})(-(int)lat[own]);")
+                w("            }")
+                w("            __syncthreads();")
+                w("        }")
+            else:
+                w(f"                if (valid) upd_{a}(Ai, lat, lat, 0, rep, own, x, y, z, key);")
+                w("            }")
+                w("        }")
+                w("        __syncthreads();")
+        w("    }")
+        w(f"    for (int i = threadIdx.x; i < {NSn}; i += {RT}) g[i] = lat[i];")
+        w("}")
+        w()
 
     # ------------------------------------------------------------------------------ packed kernels
     def emit_packed(self) -> None:
@@ -978,13 +1092,13 @@ class _Gen:
         w()
         w(f'extern "C" __global__ void __launch_bounds__(64) sck_obs_final(sck_args A) {{')
         w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
-        w("    if (rep >= A.nrep) return;")
+        w(f"    if (rep >= {self.B}) return;")
         w(f"    const int nch = {self.obs_chunks};")
         w(f"    for (int k = 0; k < {NV}; ++k) {{")
         w("        double v = 0.0;")
         w(f"        for (int c = 0; c < nch; ++c) v += A.obs_scratch[((int64_t)rep * nch + c) * {NV} + k];")
         w("        if (k == 0) A.obs[rep] = v;")
-        w(f"        else A.obs[A.nrep + (int64_t)rep * {NV - 1} + (k - 1)] = v;")
+        w(f"        else A.obs[{self.B} + (int64_t)rep * {NV - 1} + (k - 1)] = v;")
         w("    }")
         w("}")
         w()
@@ -1000,8 +1114,11 @@ class _Gen:
         self.w(f"// system: {s.name}; n={s.n} b={s.b} box={s.box} periodic={s.periodic}; "
                f"colouring {s.colouring.describe()}; mode {self.mode}; proposal {self.proposal}")
         self.w()
+        rplan = self.resident_plan()
         if self.mode in (MODE_INT8, MODE_F32):
             self.emit_plain()
+            if rplan is not None:
+                self.emit_resident(rplan)
         elif self.mode == MODE_PACKED:
             self.emit_packed()
         else:
@@ -1014,7 +1131,8 @@ class _Gen:
         self.emit_observe(chunk)
         src = "\n".join(self.lines) + "\n"
         kernels = [f"sck_class_{k}" for k in range(len(self.classes))] + ["sck_seq"] + \
-                  [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"]
+                  [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"] + \
+                  (["sck_resident"] if rplan is not None else [])
         ki = {k: i for i, k in enumerate(kernels)}
         R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
         gy, gz = (R, 1) if R <= 65535 else (65535, -(-R // 65535))
@@ -1037,6 +1155,8 @@ class _Gen:
         layouts, offsets, total = ising_layout(s) if s.n == 1 else ([], [], 0)
         mod = Module(src, kernels, sweep, ref, sync, obs, bump, self.obs_chunks, self.classes,
                      table_entries=total, ising_layout=layouts)
+        if rplan is not None:
+            mod.resident = Launch(ki["sck_resident"], (self.B, 1, 1), (self.RESIDENT_THREADS, 1, 1))
         mod.key = hashlib.sha256(src.encode()).hexdigest()[:16]
         return mod
 

@@ -7,6 +7,8 @@ and a Sampler on a non-HIP device raises.
 
 Additions: `proposal="cone"` (fixed half-angle per replica, `set_width`) and `proposal="learned"` (a width table
 from sck_amd.proposal) for XY/Heisenberg, and `observe(S)` returning energy and magnetization from one kernel.
+Whole sweeps of a system whose per-replica lattice fits in LDS run in one launch of the resident kernel
+(`resident=False` keeps the per-class launches).
 The Philox counter is 64-bit (identical to the reference's int32 counter below 2^31), so plain batches beyond
 the reference's limit run; the bitwise kernels keep the int32 counter w * nsite + i and its limit.
 """
@@ -86,8 +88,9 @@ class Sampler:
                  spin_dtype: torch.dtype | None = None, packed: bool | None = None, bitwise: bool = False,
                  words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
                  bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
-                 width=None, learned=None, cache_dir=None):
+                 width=None, learned=None, cache_dir=None, resident: bool = True):
         self.sys = system
+        self.use_resident = bool(resident)
         self.bitwise = bool(bitwise)
         self.independent = bool(independent)
         if self.independent and not self.bitwise:
@@ -201,6 +204,10 @@ class Sampler:
         barr, _ = L.launches([mod.bump])
         self._keep.append(barr)
         d.bump = C.cast(barr, C.POINTER(L.LaunchC))
+        if mod.resident is not None and self.use_resident:
+            rarr, _ = L.launches([mod.resident])
+            self._keep.append(rarr)
+            d.resident = C.cast(rarr, C.POINTER(L.LaunchC))
         h = C.c_void_p()
         cache = str(cache_dir or L.CACHE).encode()
         with torch.cuda.device(self.device):
@@ -232,7 +239,7 @@ class Sampler:
         a.obs_scratch = self._obs_scratch.data_ptr()
         a.sweep_offset = 0
         a.nrep = self.W if self.mode in (G.MODE_PACKED, G.MODE_BITWISE, G.MODE_BITWISE_INDEP) else self.B
-        a.flags = 0
+        a.nsweep = 0
         return a
 
     def _check_spins(self, S: torch.Tensor) -> None:

@@ -93,6 +93,23 @@ def test_parallel_equals_sequential_chain(i):
     assert int(A.state[1]) == 2 and int(R.state[1]) == 2
 
 
+@pytest.mark.parametrize("i", [0, 2, 4, 5, 9])
+def test_resident_kernel_equals_class_launches(i):
+    """Whole sweeps in LDS (one launch for n sweeps) equal the per-class launches spin for spin."""
+    s = SMALL[i]()
+    print(f"[gpu] resident == class launches {s.name} {s.box}", flush=True)
+    A, C = Sampler(s, 5, DEV, seed=4), Sampler(s, 5, DEV, seed=4, resident=False)
+    assert A.module.resident is not None
+    for X in (A, C):
+        X.set_temperature(torch.tensor([0.5, 1.0, 2.0, 3.0, 5.0]))
+    S1 = A.init_spins(seed=7)
+    S2 = S1.clone()
+    A.sweep(S1, 3)
+    C.sweep(S2, 3)
+    assert torch.equal(S1, S2)
+    assert int(A.state[1]) == 3
+
+
 def test_masked_sites_stay_zero_and_unit_norm_elsewhere():
     s = sy.nanodot(Rd=2.0, L=2)
     A = Sampler(s, 2, DEV, seed=3)
@@ -109,15 +126,17 @@ def test_ising_c1_shape_bit_exact_against_oracle():
     """Ising square 64^2, int8, B = 512 (8 temperatures x 64 replicas in [1.5, 3.5]), seed 0: 30 sweeps on the GPU,
     replicas 0, 1, 200, 447, 511 followed by the sequential oracle with the same counters."""
     s = sy.ising("square", 64)
-    A = Sampler(s, 512, DEV, seed=0, packed=False)
     T = np.repeat(np.linspace(1.5, 3.5, 8).astype(np.float32), 64)
-    A.set_temperature(torch.as_tensor(T))
-    S = A.init_spins(seed=0)
-    S0 = S.cpu().numpy()
-    A.sweep(S, 30)
-    reps = [0, 1, 200, 447, 511]
-    ref, _ = O.chain(O.OracleSystem(s), S0[reps].astype(np.float32), A.beta.cpu().numpy(), 0, 0, 30, reps=reps)
-    assert np.array_equal(S.cpu().numpy()[reps].astype(np.float32), ref)
+    for resident in (True, False):
+        A = Sampler(s, 512, DEV, seed=0, packed=False, resident=resident)
+        assert (A.module.resident is not None)
+        A.set_temperature(torch.as_tensor(T))
+        S = A.init_spins(seed=0)
+        S0 = S.cpu().numpy()
+        A.sweep(S, 30)
+        reps = [0, 1, 200, 447, 511]
+        ref, _ = O.chain(O.OracleSystem(s), S0[reps].astype(np.float32), A.beta.cpu().numpy(), 0, 0, 30, reps=reps)
+        assert np.array_equal(S.cpu().numpy()[reps].astype(np.float32), ref), f"resident={resident}"
 
 
 @pytest.mark.parametrize("acceptance", ["metropolis", "barker"])
