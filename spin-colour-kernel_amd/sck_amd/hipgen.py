@@ -846,6 +846,101 @@ class _Gen:
         s = self.s
         return [i for i, bd in enumerate(s.crystal.bonds) if bd.alpha == a and s.J.get(bd.shell, 0.0) != 0.0]
 
+    BW_ROUNDS = 4        # class slots per thread of an independent-mode block (fewer when queue items are large)
+
+    def bw_rounds(self, a: int) -> int:
+        item = 3 + 2 * max(1, self.bw_ndyn[a])
+        return max(1, min(self.BW_ROUNDS, (48 * 1024) // (BLOCK * item * 4)))
+
+    def emit_bw_indep_class(self, k: int, c: int, a: int) -> None:
+        """Independent-mode class kernel with a compacted tail. Every lane of a block first runs level groups
+        0 and 1 (two Philox calls: 8 of the 32 levels), which decide all 32 replicas of most (word, site)
+        items; the few items still undecided are queued in LDS with their entries, and the block then runs
+        the remaining level groups on dense waves of queued items. The draws and the result are those of the
+        plain loop: groups are taken in order and stop only when nothing is undecided."""
+        w = self.w
+        nsite = self.NS
+        nd = self.bw_ndyn[a]
+        item = 3 + 2 * max(1, nd)
+        R = self.bw_rounds(a)
+        w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
+        w(f"    __shared__ uint32_t q[{R * BLOCK}][{item}];")
+        w("    __shared__ int qn;")
+        w("    if (threadIdx.x == 0) qn = 0;")
+        w("    __syncthreads();")
+        w("    const int wd = blockIdx.y;")
+        w("    const uint32_t key = sweep_key(A);")
+        w(f"    uint32_t *S = (uint32_t *)A.spins;")
+        w(f"    uint32_t *Sw = S + (int64_t)wd * {nsite};")
+        w("    const int lane = threadIdx.x & 63;")
+        w(f"    for (int r = 0; r < {R}; ++r) {{")
+        w(f"        const int t = (blockIdx.x * {R} + r) * blockDim.x + threadIdx.x;")
+        w(f"        const int row = t / {self.BX};")
+        w(f"        const int j = t - row * {self.BX};")
+        w(f"        const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+        w(f"        bool live = row < {self.ROWS};")
+        w(f"        int x0 = {self.P};")
+        for qq in range(self.P - 1, -1, -1):
+            w(f"        if ({self.colour_expr(str(qq), a)} == {c}) x0 = {qq};")
+        w(f"        const int x = x0 + {self.P} * j;")
+        w(f"        live = live && x0 < {self.P} && x < {self.L1};")
+        w(f"        const int own = live ? {a * self.PL} + row * {self.L1} + x : 0;")
+        if self.has_mask:
+            w("        live = live && A.mask[own];")
+        w("        uint32_t s0;")
+        w(f"        BW{a} st = setup_{a}(A, S, wd, own, live ? x : 0, live ? y : 0, live ? z : 0, &s0);")
+        w("        if (!live) { st.acc = 0u; st.und = 0u; }")
+        w(f"        const uint32_t ctr = (uint32_t)wd * {nsite}u + (uint32_t)own;")
+        w(f"        levels_{a}(st, ctr, key, 0, 2);")
+        w("        const bool tail = st.und != 0u;")
+        w("        if (live && !tail) Sw[own] = s0 ^ st.acc;")
+        w("        const unsigned long long m = __ballot(tail);")
+        w("        if (m) {")
+        w("            int qb = 0;")
+        w("            if (lane == 0) qb = atomicAdd(&qn, __popcll(m));")
+        w("            qb = __shfl(qb, 0, 64);")
+        w("            if (tail) {")
+        w("                const int pos = qb + __popcll(m & ((1ull << lane) - 1ull));")
+        w("                q[pos][0] = (uint32_t)own; q[pos][1] = st.acc; q[pos][2] = st.und;")
+        for i in range(nd):
+            w(f"                q[pos][{3 + i}] = st.e[{i}]; q[pos][{3 + nd + i}] = st.P[{i}];")
+        w("            }")
+        w("        }")
+        w("    }")
+        w("    __syncthreads();")
+        w("    const int n = qn;")
+        w(f"    for (int i = threadIdx.x; i < ((n + 63) & ~63); i += {BLOCK}) {{")
+        w("        const bool has = i < n;")
+        w(f"        BW{a} st;")
+        w("        const int own = has ? (int)q[i][0] : 0;")
+        w("        st.acc = has ? q[i][1] : 0u; st.und = has ? q[i][2] : 0u;")
+        for i in range(nd):
+            w(f"        st.e[{i}] = has ? q[i][{3 + i}] : 0u; st.P[{i}] = has ? q[i][{3 + nd + i}] : 0u;")
+        w(f"        levels_{a}(st, (uint32_t)wd * {nsite}u + (uint32_t)own, key, 2, {LEVELS // 4});")
+        w("        if (has) Sw[own] ^= st.acc;")
+        w("    }")
+        w("}")
+        w()
+
+    def bw_entries(self, a: int):
+        """Flip-mask entries of the independent mode, one per k (and per spin sign with a field), as
+        (kind, lane-mask expression, table offset): 'static' entries (Metropolis on a periodic unmasked box
+        with dE <= 0, p = 1) accept with no draw (SCK/bitwise.py:373-384), 'dyn' entries compare U with P."""
+        s = self.s
+        J, h, Z = self.bitwise_params()
+        Z1 = Z + 1
+        field = h != 0.0
+        static = s.acceptance != "barker" and s.mask is None and all(s.periodic)
+        out = []
+        for k in range(Z1):
+            sels = ([(f"(eq{k}) & ~s0", k, -1), (f"(eq{k}) & s0", Z1 * Z1 + k, 1)] if field else [(f"eq{k}", k, -1)])
+            for sel, off, sigma in sels:
+                if static and k <= Z and 2.0 * J * (Z - 2 * k) + 2.0 * h * sigma <= 0.0:
+                    out.append(("static", sel, off))
+                else:
+                    out.append(("dyn", sel, off))
+        return out
+
     def emit_bitwise(self) -> None:
         """Bitwise multi-spin Ising (SCK/bitwise.py:250-422). A lane owns one site of one word (32 replicas):
         anti-aligned neighbour counts go into NP bit planes through a ripple adder, and the flip mask is
@@ -863,11 +958,21 @@ class _Gen:
         indep = self.mode == MODE_BITWISE_INDEP
         static = (indep and s.acceptance != "barker" and s.mask is None and all(s.periodic))
         nsite = self.NS
+        self.bw_ne, self.bw_ndyn = {}, {}
         for a in range(self.b):
-            w(f"DEV uint32_t flip_{a}(const sck_args &A, const uint32_t *__restrict__ S, int wd, int own, int x, int y, "
-              f"int z, uint32_t key) {{")
+            if indep:
+                nd = sum(1 for kind, _, _ in self.bw_entries(a) if kind != "static")
+                self.bw_ndyn[a] = nd
+                w(f"struct BW{a} {{ uint32_t acc, und; uint32_t e[{max(1, nd)}], P[{max(1, nd)}]; }};")
+                w(f"DEV BW{a} setup_{a}(const sck_args &A, const uint32_t *__restrict__ S, int wd, int own, int x, int y, "
+                  f"int z, uint32_t *s_out) {{")
+            else:
+                w(f"DEV uint32_t flip_{a}(const sck_args &A, const uint32_t *__restrict__ S, int wd, int own, int x, int y, "
+                  f"int z, uint32_t key) {{")
             w(f"    const int64_t wbase = (int64_t)wd * {nsite};")
             w("    const uint32_t s0 = S[wbase + own];")
+            if indep:
+                w("    *s_out = s0;")
             w("    int ze = 0;")
             for p in range(NP):
                 w(f"    uint32_t c{p} = 0u;")
@@ -904,28 +1009,35 @@ class _Gen:
                 w("}")
                 w()
                 continue
-            # independent mode
+            # independent mode: setup (entries and their P), then the level loop
             w(f"    const int64_t *tb = (const int64_t *)A.table + (int64_t)wd * {NT} + ze * {Z1};")
-            w("    uint32_t acc = 0u, und = 0u;")
-            ent = []
+            ent = self.bw_entries(a)
+            self.bw_ne[a] = len(ent)
+            w(f"    BW{a} st;")
+            w("    st.acc = 0u; st.und = 0u;")
             for k in range(Z1):
-                sels = ([(f"(eq{k}) & ~s0", k, -1), (f"(eq{k}) & s0", Z1 * Z1 + k, 1)] if field else [(f"eq{k}", k, -1)])
                 w(f"    const uint32_t eq{k} = {eqs[k]};")
-                for sel, off, sigma in sels:
-                    if static and k <= Z and 2.0 * J * (Z - 2 * k) + 2.0 * h * sigma <= 0.0:
-                        w(f"    acc |= {sel};")
-                        continue
-                    e = len(ent)
-                    w(f"    const uint32_t e{e} = {sel};")
-                    w(f"    const int64_t Pq{e} = tb[{off}];")
-                    w(f"    if (Pq{e} >= 4294967296LL) acc |= e{e};")
-                    w(f"    if (Pq{e} > 0 && Pq{e} < 4294967296LL) und |= e{e};")
-                    w(f"    const uint32_t P{e} = (uint32_t)Pq{e};")
-                    ent.append(off)
-            if ent:
-                w("    const uint32_t ctr = (uint32_t)wd * " + f"{nsite}u + (uint32_t)own;")
-                w(f"    for (int g = 0; g < {LEVELS // 4}; ++g) {{")
-                w("        if (g >= 2 && __ballot(und != 0u) == 0ull) break;")
+            for kind, sel, off in ent:
+                if kind == "static":
+                    w(f"    st.acc |= {sel};")
+                    continue
+            ei = 0
+            for kind, sel, off in ent:
+                if kind == "static":
+                    continue
+                w(f"    {{ const int64_t Pq = tb[{off}]; st.e[{ei}] = {sel}; st.P[{ei}] = (uint32_t)Pq;")
+                w(f"      if (Pq >= 4294967296LL) st.acc |= st.e[{ei}];")
+                w(f"      if (Pq > 0 && Pq < 4294967296LL) st.und |= st.e[{ei}]; }}")
+                ei += 1
+            w("    return st;")
+            w("}")
+            w()
+            nd = self.bw_ndyn[a]
+            # level groups [g0, g1): four levels per Philox call; a wave skips a group nobody needs
+            w(f"DEV void levels_{a}(BW{a} &st, uint32_t ctr, uint32_t key, int g0, int g1) {{")
+            if nd:
+                w("    for (int g = g0; g < g1; ++g) {")
+                w("        if (__ballot(st.und != 0u) == 0ull) break;")
                 w(f"        const uint32_t kg = (uint32_t)(((int64_t)key + (int64_t)(g + 1) * {GROUP_KEY_STEP}LL) % 2147483629LL);")
                 w("        uint32_t r0 = ctr, r1 = 0u, r2 = 0u, r3 = 0u;")
                 w("        philox10(r0, r1, r2, r3, kg, 0u);")
@@ -933,17 +1045,26 @@ class _Gen:
                 w("#pragma unroll")
                 w("        for (int q = 0; q < 4; ++q) {")
                 w(f"            const uint32_t sh = {LEVELS - 1} - (4 * g + q);")
-                pb = " | ".join(f"(e{i} & (0u - ((P{i} >> sh) & 1u)))" for i in range(len(ent)))
+                pb = " | ".join(f"(st.e[{i}] & (0u - ((st.P[{i}] >> sh) & 1u)))" for i in range(nd))
                 w(f"            const uint32_t pb = {pb};")
-                w("            acc |= und & ~rw[q] & pb;")
-                w("            und &= ~(rw[q] ^ pb);")
+                w("            st.acc |= st.und & ~rw[q] & pb;")
+                w("            st.und &= ~(rw[q] ^ pb);")
                 w("        }")
                 w("    }")
-            w("    return acc;")
+            w("}")
+            w()
+            w(f"DEV uint32_t flip_{a}(const sck_args &A, const uint32_t *__restrict__ S, int wd, int own, int x, int y, "
+              f"int z, uint32_t key) {{")
+            w(f"    uint32_t s0; BW{a} st = setup_{a}(A, S, wd, own, x, y, z, &s0);")
+            w(f"    levels_{a}(st, (uint32_t)wd * {nsite}u + (uint32_t)own, key, 0, {LEVELS // 4});")
+            w("    return st.acc;")
             w("}")
             w()
         # class kernels: lane = site of the class row tile, blockIdx.y = word
         for k, (c, a) in enumerate(self.classes):
+            if indep:
+                self.emit_bw_indep_class(k, c, a)
+                continue
             w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
             w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
             w("    const int wd = blockIdx.y;")
@@ -960,11 +1081,7 @@ class _Gen:
             if self.has_mask:
                 w("    live = live && A.mask[own];")
             if indep:
-                # every lane joins the wave-wide exit test of the level loop; dead lanes have nothing undecided
-                w("    uint32_t *S = (uint32_t *)A.spins;")
-                w("    const uint32_t key = sweep_key(A);")
-                w(f"    const uint32_t f = flip_{a}(A, S, wd, own, live ? x : 0, live ? y : 0, live ? z : 0, key);")
-                w(f"    if (live) S[(int64_t)wd * {nsite} + own] ^= f;")
+                raise AssertionError("independent class kernels are emitted by emit_bw_indep_class")
             else:
                 w("    if (!live) return;")
                 w("    uint32_t *S = (uint32_t *)A.spins;")
@@ -1140,6 +1257,9 @@ class _Gen:
         for k in range(len(self.classes)):
             if self.mode == MODE_PACKED:
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // (BLOCK // 32)), self.W, 1)))
+            elif self.mode == MODE_BITWISE_INDEP:
+                tile = BLOCK * self.bw_rounds(self.classes[k][1])
+                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // tile), self.W, 1)))
             else:
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // BLOCK), gy, gz)))
         ref = [Launch(ki["sck_seq"], (-(-R // 64), 1, 1), (64, 1, 1))]
