@@ -212,8 +212,8 @@ DEV void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
         const uint64_t p2 = (uint64_t)0xD2511F53u * (uint64_t)c0;
-        const uint32_t n0 = (uint32_t)(p0 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p2 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c1, k0, 0x96);   // 3-input xor
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p2 >> 32), c3, k1, 0x96);
         c1 = (uint32_t)p0;
         c3 = (uint32_t)p2;
         c0 = n0;
@@ -237,6 +237,28 @@ DEV uint32_t sweep_key(const sck_args &A) {
     return (uint32_t)((base * 1000003LL + sweep * 7919LL + 12345LL) % 2147483629LL);
 }
 
+// sin and cos of a bounded argument |x| <= 16: Cody-Waite reduction by pi/2 with a two-part constant, then the
+// Cephes single-precision minimax polynomials on [-pi/4, pi/4] (S. Moshier, Cephes Math Library, sinf.c).
+DEV void sincos_bounded(float x, float *s_out, float *c_out) {
+    const float k = __builtin_rintf(x * 0.63661977236758134f);
+    float y = __builtin_fmaf(-k, 1.5707963705062866f, x);
+    y = __builtin_fmaf(-k, -4.3711388286737929e-08f, y);
+    const float y2 = y * y;
+    float sp = __builtin_fmaf(y2, -1.9515295891e-4f, 8.3321608736e-3f);
+    sp = __builtin_fmaf(y2, sp, -1.6666654611e-1f);
+    const float s = __builtin_fmaf(y * y2, sp, y);
+    float cp = __builtin_fmaf(y2, 2.443315711809948e-5f, -1.388731625493765e-3f);
+    cp = __builtin_fmaf(y2, cp, 4.166664568298827e-2f);
+    const float c = __builtin_fmaf(y2 * y2, cp, __builtin_fmaf(-0.5f, y2, 1.0f));
+    const int q = (int)k & 3;
+    const float ss = (q & 1) ? c : s, cc = (q & 1) ? s : c;
+    *s_out = (q & 2) ? -ss : ss;
+    *c_out = ((q + 1) & 2) ? -cc : cc;
+}
+
+// e^x through the hardware base-2 exponential
+DEV float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
 DEV int wrap_add(int v, int d, int L) {
     v += d;
     v = v >= L ? v - L : v;
@@ -246,7 +268,7 @@ DEV int wrap_add(int v, int d, int L) {
 
 
 class _Gen:
-    def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None):
+    def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False):
         self.s = sys_
         self.mode = mode
         self.B = int(B)
@@ -266,9 +288,27 @@ class _Gen:
         self.P = x_period(sys_.colouring)
         self.BX = -(-self.L1 // self.P)
         self.wide = self.Bp * self.NS >= 2 ** 31    # 64-bit Philox counter arithmetic needed
+        # replica-major storage [b][L3][L2][L1][n][B] (lanes over replicas) or the reference layout
+        self.replica_major = bool(replica_major) and mode in (MODE_INT8, MODE_F32)
+        self.SS = self.B if self.replica_major else 1
         self.lines: list[str] = []
 
     # -------------------------------------------------------------------------------------- helpers
+    def at(self, ptr: str, comp: int, site: str) -> str:
+        """Component `comp` of site `site` of the replica whose offset is `rbase`. Replica-major storage is
+        [site][n][B]: a wave-uniform byte address of the site (scalar) plus the lane's 32-bit offset
+        (replica and component), so each load takes a scalar base and a vector offset."""
+        if self.replica_major:
+            esz = 1 if self.mode == MODE_INT8 else 4
+            return (f"(*(decltype({ptr}))((const char *)({ptr}) + (int64_t)({site}) * {self.n * self.B * esz} + "
+                    f"((uint32_t)rbase + {comp * self.B}u) * {esz}u))")
+        return f"{ptr}[rbase + (int64_t)({comp * self.NS} + {site})]"
+
+    def rbase_expr(self, rep: str) -> str:
+        """Offset of replica `rep`'s first spin: the address of component k of site q is
+        rbase + (k * NS + q) * SS."""
+        return f"(int64_t){rep}" if self.replica_major else f"(int64_t){rep} * {self.n * self.NS}"
+
     def w(self, s: str = "") -> None:
         self.lines.append(s)
 
@@ -333,7 +373,7 @@ class _Gen:
             # table lookup: p (Metropolis) or 1 + exp(beta dE) (Barker) of the site's neighbour configuration
             w(f"DEV float isp_{a}(const {T} *__restrict__ src, int64_t rbase, int own, int x, int y, int z, "
               f"const float *__restrict__ tb, int *s_out) {{")
-            w(f"    const int s0 = (int)src[rbase + own];")
+            w(f"    const int s0 = (int){self.at('src', 0, 'own')};")
             w("    *s_out = s0;")
             ng = sum(1 for Jv, _ in groups if Jv != 0.0)
             for k in range(ng):
@@ -346,7 +386,7 @@ class _Gen:
                 w("        int g = 0;")
                 for i in members:
                     q, ok = self.nb_index(i)
-                    w(f"        {{ const int q = {q}; const int t = ({ok}) ? (int)src[rbase + q] : 0; g += t;"
+                    w(f"        {{ const int q = {q}; const int t = ({ok}) ? (int){self.at('src', 0, 'q')} : 0; g += t;"
                       + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
                 if Jv != 0.0:
                     w(f"        gsum{gi} = g + {len(members)};")
@@ -378,7 +418,7 @@ class _Gen:
             self.emit_draw("        ")
             w(f"        acc = {self.accept_expr('p')};")
             w("    }")
-            w(f"    if (acc) dst[rbase + own] = ({T})(-s0);")
+            w(f"    if (acc) {self.at('dst', 0, 'own')} = ({T})(-s0);")
             w("    return acc;")
             w("}")
             return
@@ -386,7 +426,7 @@ class _Gen:
           f"int rep, int own, int x, int y, int z, uint32_t key) {{")
         # vector models: fp32 local field in the reference's order
         for k in range(n):
-            w(f"    float s{k} = (float)src[rbase + {k * self.NS} + own];")
+            w(f"    float s{k} = (float){self.at('src', k, 'own')};")
         for k in range(n):
             w(f"    float v{k} = 0.0f;")
         for j in range(len(extras)):
@@ -400,7 +440,7 @@ class _Gen:
                 q, ok = self.nb_index(i)
                 w(f"        {{ const int q = {q}; const bool ok = {ok};")
                 for k in range(n):
-                    w(f"          const float t{k} = ok ? (float)src[rbase + {k * self.NS} + q] : 0.0f; g{k} = g{k} + t{k};")
+                    w(f"          const float t{k} = ok ? (float){self.at('src', k, 'q')} : 0.0f; g{k} = g{k} + t{k};")
                     if i in extras:
                         w(f"          n{extras.index(i)}_{k} = t{k};")
                 w("        }")
@@ -449,12 +489,12 @@ class _Gen:
             w("      dE = dE + ku * (sn * sn - tn * tn); }")
         w("    const float beta = A.beta[rep];")
         if s.acceptance == "barker":
-            w("    const int acc = (u0 * (1.0f + expf(beta * dE))) < 1.0f;")
+            w("    const int acc = (u0 * (1.0f + exp_fast(beta * dE))) < 1.0f;")
         else:
-            w("    const int acc = u0 < expf((-beta) * dE);")
+            w("    const int acc = u0 < exp_fast((-beta) * dE);")
         w("    if (acc) {")
         for k in range(n):
-            w(f"        dst[rbase + {k * self.NS} + own] = t{k};")
+            w(f"        {self.at('dst', k, 'own')} = t{k};")
         w("    }")
         w("    return acc;")
         w("}")
@@ -486,12 +526,12 @@ class _Gen:
         if self.proposal == "uniform":
             if n == 2:
                 w(f"    const float phi = {TWO_PI} * u1;")
-                w("    float t0, t1; sincosf(phi, &t1, &t0);")
+                w("    float t0, t1; sincos_bounded(phi, &t1, &t0);")
             else:
                 w("    const float cz = 2.0f * u1 - 1.0f;")
                 w(f"    const float phi = {TWO_PI} * u2;")
                 w("    const float rr = sqrtf(fmaxf(1.0f - cz * cz, 0.0f));")
-                w("    float sp, cp; sincosf(phi, &sp, &cp);")
+                w("    float sp, cp; sincos_bounded(phi, &sp, &cp);")
                 w("    const float t0 = rr * cp, t1 = rr * sp, t2 = cz;")
             return
         # cone proposal
@@ -509,9 +549,10 @@ class _Gen:
             w(f"    const float wd = A.width[((int64_t)rep * {lm['n0']} + i0) * {lm['n1']} + i1];")
         if n == 2:
             w("    const float ph = atan2f(s1, s0) + wd * (2.0f * u1 - 1.0f);")
-            w("    float t0, t1; sincosf(ph, &t1, &t0);")
+            w("    float t0, t1; sincos_bounded(ph, &t1, &t0);")
         else:
-            w("    const float cc = 1.0f - u1 * (1.0f - cosf(wd));")
+            w("    float sw, cw; sincos_bounded(wd, &sw, &cw);")
+            w("    const float cc = 1.0f - u1 * (1.0f - cw);")
             w("    const float st = sqrtf(fmaxf(1.0f - cc * cc, 0.0f));")
             w(f"    const float phi = {TWO_PI} * u2;")
             w("    const bool nz = fabsf(s2) > 0.9f;")
@@ -520,7 +561,7 @@ class _Gen:
             w("    const float ne = sqrtf(e10 * e10 + e11 * e11 + e12 * e12);")
             w("    e10 = e10 / ne; e11 = e11 / ne; e12 = e12 / ne;")
             w("    const float e20 = s1 * e12 - s2 * e11, e21 = s2 * e10 - s0 * e12, e22 = s0 * e11 - s1 * e10;")
-            w("    float sp, cp; sincosf(phi, &sp, &cp);")
+            w("    float sp, cp; sincos_bounded(phi, &sp, &cp);")
             w("    float t0 = cc * s0 + st * (cp * e10 + sp * e20);")
             w("    float t1 = cc * s1 + st * (cp * e11 + sp * e21);")
             w("    float t2 = cc * s2 + st * (cp * e12 + sp * e22);")
@@ -536,6 +577,9 @@ class _Gen:
             w()
         # class kernels
         for k, (c, a) in enumerate(self.classes):
+            if self.replica_major:
+                self.emit_rm_class(k, c, a, T)
+                continue
             w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
             w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
             w(f"    const int row = t / {self.BX};")
@@ -561,7 +605,7 @@ class _Gen:
         w("    if (rep >= A.nrep) return;")
         w("    const uint32_t key = sweep_key(A);")
         w(f"    {T} *S = ({T} *)A.spins;")
-        w(f"    const int64_t rbase = (int64_t)rep * {self.n * self.NS};")
+        w(f"    const int64_t rbase = {self.rbase_expr('rep')};")
         for c, a in self.classes:
             w(f"    for (int z = 0; z < {self.L3}; ++z) for (int y = 0; y < {self.L2}; ++y) for (int x = 0; x < {self.L1}; ++x) {{")
             w(f"        if ({self.colour_expr('x', a)} != {c}) continue;")
@@ -575,6 +619,20 @@ class _Gen:
         # synchronous control: every site of sublattice a from the old state
         for a in range(self.b):
             w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_sync_{a}(sck_args A) {{')
+            if self.replica_major:
+                w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
+                w("    const int t = blockIdx.y + blockIdx.z * gridDim.y;")
+                w(f"    if (t >= {self.PL} || rep >= {self.B}) return;")
+                w(f"    const int x = t % {self.L1}, row = t / {self.L1};")
+                w(f"    const int z = row / {self.L2}, y = row % {self.L2};")
+                w(f"    const int own = {a * self.PL} + t;")
+                if self.has_mask:
+                    w("    if (!A.mask[own]) return;")
+                w("    const uint32_t key = sweep_key(A);")
+                w(f"    upd_{a}(A, (const {T} *)A.spins, ({T} *)A.dst, (int64_t)rep, rep, own, x, y, z, key);")
+                w("}")
+                w()
+                continue
             w("    const int t = blockIdx.x * blockDim.x + threadIdx.x;")
             w(f"    if (t >= {self.PL}) return;")
             w(f"    const int x = t % {self.L1}, row = t / {self.L1};")
@@ -585,10 +643,44 @@ class _Gen:
             w("    const int rep = blockIdx.y + blockIdx.z * gridDim.y;")
             w("    if (rep >= A.nrep) return;")
             w("    const uint32_t key = sweep_key(A);")
-            w(f"    const int64_t rbase = (int64_t)rep * {self.n * self.NS};")
+            w(f"    const int64_t rbase = {self.rbase_expr('rep')};")
             w(f"    upd_{a}(A, (const {T} *)A.spins, ({T} *)A.dst, rbase, rep, own, x, y, z, key);")
             w("}")
             w()
+
+    def emit_rm_class(self, k: int, c: int, a: int, T: str) -> None:
+        """Replica-major class kernel: the 64 lanes of a wave are 64 consecutive replicas of one class site, so
+        the site, its colour residue and every neighbour index are wave-uniform (scalar) and each spin load is
+        one coalesced 256-byte access. blockIdx.x is the replica group (fastest, so a group stays on one XCD
+        and its L2), blockIdx.y/z the class slot."""
+        w = self.w
+        R = self.rm_chunks()
+        w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
+        w(f"    const int rep0 = blockIdx.x * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
+        w("    const int t = blockIdx.y + blockIdx.z * gridDim.y;")
+        w(f"    const int row = t / {self.BX};")
+        w(f"    if (row >= {self.ROWS}) return;")
+        w(f"    const int j = t - row * {self.BX};")
+        w(f"    const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+        self.emit_residue(c, a)
+        w(f"    const int x = x0 + {self.P} * j;")
+        w(f"    if (x >= {self.L1}) return;")
+        w(f"    const int own = {a * self.PL} + row * {self.L1} + x;")
+        if self.has_mask:
+            w("    if (!A.mask[own]) return;")
+        w("    const uint32_t key = sweep_key(A);")
+        w(f"    {T} *S = ({T} *)A.spins;")
+        # one wave, one site, R chunks of 64 replicas: the site's scalar address work is done once
+        w(f"    for (int i = 0; i < {R}; ++i) {{")
+        w("        const int rep = rep0 + 64 * i;")
+        w(f"        if (rep < {self.B}) upd_{a}(A, S, S, (int64_t)rep, rep, own, x, y, z, key);")
+        w("    }")
+        w("}")
+        w()
+
+    def rm_chunks(self) -> int:
+        """64-replica chunks one wave of a replica-major class kernel runs for its site."""
+        return max(1, min(4, self.B // BLOCK))
 
     def emit_residue(self, c: int, a: int) -> None:
         """x0: the x residue in [0, P) of row (y, z) where sublattice a has colour c (P when none)."""
@@ -604,7 +696,7 @@ class _Gen:
 
     def resident_plan(self):
         """(chunk, LDS bytes) of the LDS-resident kernel, or None when one replica's lattice does not fit."""
-        if self.mode not in (MODE_INT8, MODE_F32):
+        if self.mode not in (MODE_INT8, MODE_F32) or self.replica_major:
             return None
         elt = 1 if self.mode == MODE_INT8 else 4
         lat = self.n * self.NS * elt
@@ -1137,10 +1229,12 @@ class _Gen:
             w(f"    const uint32_t wv = ((const uint32_t *)A.spins)[(int64_t)(rep >> 5) * {self.NS} + q];")
             w("    const double sv = ((wv >> (rep & 31)) & 1u) ? 1.0 : -1.0;")
             w("    return A.mask[q] ? sv : 0.0;")
-        elif self.mode == MODE_INT8:
-            w(f"    return (double)((const int8_t *)A.spins)[(int64_t)rep * {n * self.NS} + (int64_t)comp * {self.NS} + q];")
         else:
-            w(f"    return (double)((const float *)A.spins)[(int64_t)rep * {n * self.NS} + (int64_t)comp * {self.NS} + q];")
+            T = "int8_t" if self.mode == MODE_INT8 else "float"
+            if self.replica_major:
+                w(f"    return (double)((const {T} *)A.spins)[((int64_t)q * {self.n} + comp) * {self.B} + rep];")
+            else:
+                w(f"    return (double)((const {T} *)A.spins)[(int64_t)rep * {self.n * self.NS} + (int64_t)comp * {self.NS} + q];")
         w("}")
         w()
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_obs(sck_args A) {{')
@@ -1260,6 +1354,10 @@ class _Gen:
             elif self.mode == MODE_BITWISE_INDEP:
                 tile = BLOCK * self.bw_rounds(self.classes[k][1])
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // tile), self.W, 1)))
+            elif self.replica_major:
+                slots = self.ROWS * self.BX
+                gy = min(slots, 65535)
+                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.B // (BLOCK * self.rm_chunks())), gy, -(-slots // gy))))
             else:
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // BLOCK), gy, gz)))
         ref = [Launch(ki["sck_seq"], (-(-R // 64), 1, 1), (64, 1, 1))]
@@ -1267,6 +1365,9 @@ class _Gen:
         for a in range(self.b):
             if self.mode == MODE_PACKED:
                 sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.PL // (BLOCK // 32)), self.W, 1)))
+            elif self.replica_major:
+                gy2 = min(self.PL, 65535)
+                sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.B // BLOCK), gy2, -(-self.PL // gy2))))
             else:
                 sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.PL // BLOCK), gy, gz)))
         obs = [Launch(ki["sck_obs"], (self.obs_chunks, self.B, 1)),
@@ -1281,11 +1382,17 @@ class _Gen:
         return mod
 
 
-def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None) -> Module:
+def fits_resident(sys_: System, mode: int) -> bool:
+    """Whether one replica's lattice fits the LDS-resident kernel."""
+    return _Gen(sys_, mode, 1, "uniform").resident_plan() is not None
+
+
+def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
+             replica_major: bool = False) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _Gen(sys_, mode, batch, proposal, learned_meta).generate()
+    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major).generate()

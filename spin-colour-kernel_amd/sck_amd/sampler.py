@@ -88,7 +88,7 @@ class Sampler:
                  spin_dtype: torch.dtype | None = None, packed: bool | None = None, bitwise: bool = False,
                  words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
                  bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
-                 width=None, learned=None, cache_dir=None, resident: bool = True):
+                 width=None, learned=None, cache_dir=None, resident: bool = True, layout: str = "auto"):
         self.sys = system
         self.use_resident = bool(resident)
         self.bitwise = bool(bitwise)
@@ -143,7 +143,20 @@ class Sampler:
         else:
             self.mode = G.MODE_INT8 if self.spin_dtype == torch.int8 else G.MODE_F32
         self.learned_meta = learned.meta() if learned is not None else None
-        self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta)
+        if layout not in ("auto", "reference", "replica"):
+            raise ValueError("layout must be 'auto', 'reference' or 'replica'")
+        rm_ok = self.mode in (G.MODE_INT8, G.MODE_F32)
+        if layout == "replica" and not rm_ok:
+            raise ValueError("the replica-major layout holds unpacked spins only")
+        if layout == "auto":
+            # lanes over replicas need whole waves of replicas; a lattice that fits LDS runs resident instead
+            layout = "replica" if (rm_ok and self.B % 64 == 0 and not (self.use_resident and
+                                                                       G.fits_resident(system, self.mode))) \
+                else "reference"
+        self.replica_major = layout == "replica"
+        self.layout = layout
+        self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta,
+                                 replica_major=self.replica_major)
         self.src = self.module.src
         dev = self.device
         m = system.site_mask()
@@ -250,8 +263,35 @@ class Sampler:
             shape, dt = (self.B, s.n) + s.shape, self.spin_dtype
         if tuple(S.shape) != shape or S.dtype != dt:
             raise ValueError(f"spins must be {dt} of shape {shape}, got {S.dtype} {tuple(S.shape)}")
-        if S.device != self.device or not S.is_contiguous():
-            raise ValueError(f"spins must be a contiguous tensor on {self.device}")
+        if S.device != self.device:
+            raise ValueError(f"spins must be on {self.device}")
+
+    def _is_native(self, S: torch.Tensor) -> bool:
+        """Spins in the storage order the kernels index: the reference layout, or for replica-major samplers
+        the (B, n, b, L3, L2, L1) view of [n][b][L3][L2][L1][B] storage that init_spins returns."""
+        if not self.replica_major:
+            return S.is_contiguous()
+        return S.permute(2, 3, 4, 5, 1, 0).is_contiguous()
+
+    def _native(self, S: torch.Tensor) -> torch.Tensor:
+        """S in the kernels' storage order: contiguous, or (replica-major) the (B, n, b, L3, L2, L1) view of
+        contiguous [b][L3][L2][L1][n][B] storage."""
+        if self._is_native(S):
+            return S
+        if not self.replica_major:
+            return S.contiguous()
+        return S.permute(2, 3, 4, 5, 1, 0).contiguous().permute(5, 4, 0, 1, 2, 3)
+
+    def _run(self, S: torch.Tensor, fn) -> torch.Tensor:
+        """fn(native spins) in place on S, through a native-order copy when S is in the other order."""
+        self._check_spins(S)
+        if self._is_native(S):
+            fn(S)
+            return S
+        T = self._native(S)
+        fn(T)
+        S.copy_(T)
+        return S
 
     # ------------------------------------------------------------------------------------ state
     def set_temperature(self, T):
@@ -308,7 +348,7 @@ class Sampler:
         x = x.to(self.device) * self._mask_f
         if self.packed:
             return self.pack(x)
-        return x.to(self.spin_dtype).contiguous()
+        return self._native(x.to(self.spin_dtype).contiguous())
 
     def pack(self, x: torch.Tensor) -> torch.Tensor:
         """(B, 1, b, L3, L2, L1) spins of +-1 -> (W, 1, b, L3, L2, L1) int32 words (SCK/sampler.py:169-178)."""
@@ -335,18 +375,21 @@ class Sampler:
     def sweep(self, S: torch.Tensor, n: int = 1, classes: tuple[int, int] | None = None) -> torch.Tensor:
         """n coloured sweeps in place; advances the sweep counter (SCK/sampler.py:234-240). `classes=(lo, hi)`
         restricts each sweep to class launches [lo, hi) (per-class checks)."""
-        self._check_spins(S)
         lo, hi = classes if classes is not None else (0, -1)
-        a = self._args(S)
-        with torch.cuda.device(self.device):
-            L.check(L.lib().sck_sweep(self._handle, C.byref(a), int(n), int(lo), int(hi),
-                                      C.c_void_p(_stream_ptr(self.device))), "sweep")
-        return S
+
+        def run(T):
+            a = self._args(T)
+            with torch.cuda.device(self.device):
+                L.check(L.lib().sck_sweep(self._handle, C.byref(a), int(n), int(lo), int(hi),
+                                          C.c_void_p(_stream_ptr(self.device))), "sweep")
+        return self._run(S, run)
 
     def graph(self, S: torch.Tensor, k: int, observe: bool = False):
         """Capture k sweeps (and, with observe=True, one observation after them) in a HIP graph; returns its
         replay (SCK/sampler.py:242-269). The replay keeps the sampler and S alive."""
         self._check_spins(S)
+        if not self._is_native(S):
+            raise ValueError("graph capture needs spins in the sampler's storage order (as init_spins returns them)")
         a = self._args(S)
         g = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -382,29 +425,30 @@ class Sampler:
 
     def sync_sweep(self, S: torch.Tensor, n: int = 1) -> torch.Tensor:
         """The synchronous control: every site updated from the old state (SCK/sampler.py:271-279)."""
-        self._check_spins(S)
-        dst = torch.empty_like(S)
-        a = self._args(S, dst)
-        with torch.cuda.device(self.device):
-            L.check(L.lib().sck_sync_sweep(self._handle, C.byref(a), int(n), S.numel() * S.element_size(),
-                                           C.c_void_p(_stream_ptr(self.device))), "sync sweep")
-        return S
+        def run(T):
+            dst = torch.empty_like(T)
+            a = self._args(T, dst)
+            with torch.cuda.device(self.device):
+                L.check(L.lib().sck_sync_sweep(self._handle, C.byref(a), int(n), T.numel() * T.element_size(),
+                                               C.c_void_p(_stream_ptr(self.device))), "sync sweep")
+        return self._run(S, run)
 
     def reference_sweep(self, S: torch.Tensor, n: int = 1) -> torch.Tensor:
         """Sequential reference chain: sites one at a time in class order, raster order inside a class, with
         the same update and counters (SCK/sampler.py:281-293)."""
-        self._check_spins(S)
-        a = self._args(S)
-        with torch.cuda.device(self.device):
-            L.check(L.lib().sck_reference_sweep(self._handle, C.byref(a), int(n),
-                                                C.c_void_p(_stream_ptr(self.device))), "reference sweep")
-        return S
+        def run(T):
+            a = self._args(T)
+            with torch.cuda.device(self.device):
+                L.check(L.lib().sck_reference_sweep(self._handle, C.byref(a), int(n),
+                                                    C.c_void_p(_stream_ptr(self.device))), "reference sweep")
+        return self._run(S, run)
 
     # ------------------------------------------------------------------------------ observables
     def observe(self, S: torch.Tensor):
         """(E (B,) float64, M (B, b, n) float64) from the FP64 reduction kernels."""
         self._check_spins(S)
-        a = self._args(S)
+        T = self._native(S)          # held until the kernels are queued on this stream
+        a = self._args(T)
         with torch.cuda.device(self.device):
             L.check(L.lib().sck_observe(self._handle, C.byref(a), C.c_void_p(_stream_ptr(self.device))), "observe")
         B, b, n = self.B, self.sys.b, self.sys.n

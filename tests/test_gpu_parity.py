@@ -110,6 +110,40 @@ def test_resident_kernel_equals_class_launches(i):
     assert int(A.state[1]) == 3
 
 
+@pytest.mark.parametrize("make", [
+    lambda: sy.System("Heis bcc J1J2 D h", 3, cr.make("bcc", (1, 2)), (4, 4, 6), (True,) * 3, {1: 1.0, 2: -0.2},
+                      h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0))),
+    lambda: sy.xy("triangular", 12), lambda: sy.nanodot(Rd=3.0, L=3), lambda: sy.ising("square", 16)])
+def test_replica_major_layout_equals_reference_layout(make):
+    """Lanes over replicas ([n][site][B] storage) and lanes over x (reference layout) run the same update: equal
+    spins; a reference-layout tensor handed to a replica-major sampler is converted and written back."""
+    s = make()
+    B = 128
+    R = Sampler(s, B, DEV, seed=6, layout="replica", packed=False)
+    F = Sampler(s, B, DEV, seed=6, layout="reference", resident=False, packed=False)
+    assert R.replica_major and not F.replica_major
+    T = torch.linspace(0.4, 3.0, B)
+    for X in (R, F):
+        X.set_temperature(T)
+    S1, S2 = R.init_spins(seed=3), F.init_spins(seed=3)
+    assert not S1.is_contiguous() and torch.equal(S1, S2)
+    R.sweep(S1, 2)
+    F.sweep(S2, 2)
+    assert torch.equal(S1, S2)
+    S3 = S2.clone()
+    R.sweep(S3, 1)                                           # contiguous input: converted, then written back
+    F.sweep(S2, 1)
+    assert torch.equal(S3, S2)
+    E1, M1 = R.observe(S1)
+    E2, M2 = F.observe(S1.contiguous())
+    assert torch.allclose(E1, E2, rtol=1e-12) and torch.allclose(M1, M2, rtol=1e-12)
+    Rq = Sampler(s, B, DEV, seed=6, layout="replica", packed=False)
+    Rq.set_temperature(T)
+    S4 = Rq.init_spins(seed=3)
+    Rq.reference_sweep(S4, 2)
+    assert torch.equal(S4, S1.contiguous().view_as(S4)) or torch.equal(S4, S1)
+
+
 def test_masked_sites_stay_zero_and_unit_norm_elsewhere():
     s = sy.nanodot(Rd=2.0, L=2)
     A = Sampler(s, 2, DEV, seed=3)
