@@ -35,6 +35,8 @@ WORKLOADS = {
     "ising": "Ising square L=64 periodic, int8, B=512 (8 temps x 64 in [1.5,3.5]), seed 0",
     "heisenberg": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, fp32, B=8192 (32 temps x 256 in [0.5,3.0])",
     "xy": "XY triangular L=2048 periodic J1=1, fp32, B=128 replica block (16 temps x 8 in [0.7,1.1])",
+    "learned": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, B=8192 (32 temps in [0.5,3.0]): effective "
+               "samples/s of the learned cone width over the best fixed width",
 }
 
 
@@ -109,6 +111,31 @@ def traffic_from_profiles(workload: str):
     return None
 
 
+def learned_main(a) -> None:
+    """Effective independent samples/s of E and |M|, learned width vs the best fixed width (one JSON line)."""
+    import numpy as np
+
+    from sck_amd.ess import compare_proposals
+    s, _, B, T, _ = _make("heisenberg")
+    temps = np.unique(T.numpy())
+    meas = max(50, a.steps)
+    r = compare_proposals(s, B, temps, warmup=200, therm=max(20, a.warmup), n_meas=meas, log=lambda m: print(m, flush=True))
+    out = {"metric": "effective samples/s, learned vs fixed proposal", "value": r["gain_E"],
+           "unit": "ratio of ESS/s (energy)", "n_gpus": 1, "steps": meas, "warmup": a.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (random initial spins from the seeded torch CPU generator)",
+           "config": {"workload": WORKLOADS["learned"], "batch": B, "sites": s.n_active},
+           "gain_E": r["gain_E"], "gain_M": r["gain_M"], "fixed_width": r["fixed_width"],
+           "ess_per_s": {k: {"E": r[k]["ess_E_per_s"], "M": r[k]["ess_M_per_s"],
+                             "updates_per_s": r[k]["updates_per_s"]} for k in ("fixed", "learned")},
+           "observables_max_z": r["max_z"], "observables_frac_within_2sigma": r["frac_within_2sigma"],
+           "train_seconds": r["train_seconds"]}
+    f = REPO / "gpurun_out"
+    f.mkdir(exist_ok=True)
+    (f / "learned_comparison.json").write_text(json.dumps(r, default=float, indent=1))
+    print(json.dumps(out, default=float), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +151,9 @@ def main() -> None:
     import torch
 
     from sck_amd.sampler import Sampler
+    if a.workload == "learned":
+        learned_main(a)
+        return
     s, kw, B, T, alg_bytes = _make(a.workload)
     A = Sampler(s, B, "cuda:0", seed=0, **kw)
     A.set_temperature(T)

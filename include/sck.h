@@ -86,6 +86,41 @@ typedef struct sck_desc {
                                                     one launch performs args.nsweep sweeps                */
 } sck_desc;
 
+/* ---------------------------------------------------------------- learned proposal width (no reference
+ * counterpart: the reference proposes a uniform redraw, SCK/generator.py:274-289). */
+
+/* One training sample, taken from a warm-up configuration: a site's spin, the local field of its neighbours
+ * (exchange + DMI + field) and the site's constants. */
+typedef struct sck_mlp_sample {
+    float s[3];
+    float v[3];
+    float beta;
+    float k1;                 /* cubic K1 of the site        */
+    float ku;                 /* per-site uniaxial strength  */
+    float axis[3];            /* per-site uniaxial axis      */
+} sck_mlp_sample;
+
+typedef struct sck_mlp_config {
+    int32_t n;                /* spin components, 2 or 3                      */
+    int32_t ndraw;            /* cone draws per sample                        */
+    int32_t seed;             /* Philox key of the draws                      */
+    float wmin, wmax;         /* half-angle range of the cone                 */
+    float k2;                 /* cubic K2                                     */
+    float du;                 /* uniform uniaxial strength                    */
+    float axis[3];            /* its axis, also the feature axis e            */
+    float xs[3];              /* input scaling of the three features          */
+} sck_mlp_config;
+
+typedef struct sck_mlp_adam_config {
+    float lr, beta1, beta2, eps;
+    int32_t step;             /* 1-based Adam step; lr = 0 computes the gradient only */
+} sck_mlp_adam_config;
+
+typedef struct sck_width_grid {
+    int32_t n[3];             /* bins per feature        */
+    float s[3];               /* bins per unit feature   */
+} sck_width_grid;
+
 typedef struct sck_handle sck_handle;
 typedef struct sck_graph sck_graph;
 
@@ -128,6 +163,23 @@ int sck_graph_destroy(sck_graph *g);
 /* Elapsed milliseconds of `reps` replays of a graph (or of k sweeps when g is NULL), timed with HIP events
  * on `stream`. */
 int sck_time_replays(sck_graph *g, int32_t reps, void *stream, float *ms);
+
+/* Learned proposal: number of network parameters (3 -> 32 -> 32 -> 1), workspace bytes of one training
+ * step, one training step (forward + backward over nsample x ndraw draws, fixed-order gradient sums, Adam;
+ * grad_out receives the gradient of -mean objective and, last, the summed objective), and the width table of
+ * the frozen network on a feature grid. No counterpart in the reference. */
+int sck_mlp_nparam(void);
+int64_t sck_mlp_workspace_bytes(int32_t nsample, int32_t ndraw);
+int sck_mlp_train_step(float *params, float *adam_m, float *adam_v, const sck_mlp_sample *samples, int32_t nsample,
+                       const sck_mlp_config *cfg, const sck_mlp_adam_config *adam, void *workspace, float *grad_out,
+                       void *stream);
+int sck_mlp_table(const float *params, const sck_mlp_config *cfg, const sck_width_grid *grid, float *table,
+                  void *stream);
+
+/* Training samples from the current spins: for rep_site[2i], rep_site[2i+1] (replica, site) the module's
+ * sck_sample kernel writes out[i]. */
+int sck_sample_sites(sck_handle *h, const sck_args *args, const int32_t *rep_site, int32_t nsample,
+                     sck_mlp_sample *out, void *stream);
 
 #ifdef __cplusplus
 }

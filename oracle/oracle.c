@@ -69,11 +69,11 @@ enum { PROP_UNIFORM = 0, PROP_CONE = 1 };
 typedef struct {
     int32_t kind;
     const float *width;      /* [B] fixed cone half-angle per replica (PROP_CONE, width_table == NULL) */
-    /* learned width: w = table[rep][i0][i1] with i0 = min(int(|v| beta * s0), n0-1),
-       i1 = min(int(f2 * s1), n1-1), f2 = |v . e| / |v| (0 when |v| = 0) */
+    /* learned width: w = table[i0][i1][i2], i_d = clamp(int(f_d * s_d), 0, n_d - 1) with the features
+       f0 = |v| beta, f1 = (|Du| + |Ku_i| + |K1_i|) beta, f2 = |v . e| / |v| (0 when |v| = 0) */
     const float *width_table;
-    int32_t n0, n1;
-    float s0, s1;
+    int32_t n[3];
+    float s[3];
     float axis[3];
 } oracle_prop;
 
@@ -249,26 +249,28 @@ static float dot_n(const float *a, const float *b, int n) {
     return d;
 }
 
-/* Cone width of one update: fixed per replica, or the learned table looked up from the local field. */
-static float cone_width(const ctx_t *cx, int64_t rep, const float *v, float beta) {
+/* Cone width of one update: fixed per replica, or the learned table looked up from features of the site's
+ * environment that its own move leaves unchanged. */
+static float cone_width(const ctx_t *cx, int64_t rep, int64_t own, const float *v, float beta) {
     const oracle_prop *pp = cx->prop;
-    int n = cx->s->n;
+    const oracle_sys *s = cx->s;
+    int n = s->n;
     if (!pp->width_table) return pp->width[rep];
     float vv = dot_n(v, v, n);
     float hv = sqrtf(vv);
-    float f0 = hv * beta;
-    float f2 = 0.0f;
-    if (hv > 0.0f) {
-        float ve = dot_n(v, pp->axis, n);
-        f2 = fabsf(ve) / hv;
+    float K = s->has_uniaxial ? (float)fabs(s->du) : 0.0f;
+    if (s->has_site) K = K + fabsf(s->site_ku[own]);
+    if (s->has_cubic) K = K + fabsf(s->k1[own]);
+    float f[3];
+    f[0] = hv * beta;
+    f[1] = K * beta;
+    f[2] = hv > 0.0f ? fabsf(dot_n(v, pp->axis, n)) / hv : 0.0f;
+    int idx[3];
+    for (int d = 0; d < 3; ++d) {
+        int i = (int)(f[d] * pp->s[d]);
+        idx[d] = i > pp->n[d] - 1 ? pp->n[d] - 1 : (i < 0 ? 0 : i);
     }
-    int i0 = (int)(f0 * pp->s0);
-    int i1 = (int)(f2 * pp->s1);
-    if (i0 > pp->n0 - 1) i0 = pp->n0 - 1;
-    if (i1 > pp->n1 - 1) i1 = pp->n1 - 1;
-    if (i0 < 0) i0 = 0;
-    if (i1 < 0) i1 = 0;
-    return pp->width_table[(rep * pp->n0 + i0) * pp->n1 + i1];
+    return pp->width_table[(idx[0] * pp->n[1] + idx[1]) * pp->n[2] + idx[2]];
 }
 
 /* The Metropolis/Barker update of one site of one replica. `src` is read for the local field, `own` is
@@ -304,7 +306,7 @@ static int update_site(const ctx_t *cx, const float *src, float *dst, int64_t ns
             t[2] = cz;
         }
     } else {
-        float w = cone_width(cx, rep, v, beta);
+        float w = cone_width(cx, rep, own, v, beta);
         if (n == 2) {
             float ph = atan2f(sv[1], sv[0]) + w * (2.0f * u1 - 1.0f);
             t[0] = cosf(ph);

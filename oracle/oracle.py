@@ -55,8 +55,8 @@ class _Sys(C.Structure):
 
 
 class _Prop(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("width", C.c_void_p), ("width_table", C.c_void_p), ("n0", C.c_int32),
-                ("n1", C.c_int32), ("s0", C.c_float), ("s1", C.c_float), ("axis", C.c_float * 3)]
+    _fields_ = [("kind", C.c_int32), ("width", C.c_void_p), ("width_table", C.c_void_p), ("n", C.c_int32 * 3),
+                ("s", C.c_float * 3), ("axis", C.c_float * 3)]
 
 
 def _ptr(a: np.ndarray) -> int:
@@ -133,7 +133,7 @@ def numpy_exp32(x: float) -> float:
 _EXP_CB = C.CFUNCTYPE(C.c_float, C.c_float)(numpy_exp32)
 
 
-def _prop(kind: int = PROP_UNIFORM, width=None, table=None, n0=0, n1=0, s0=0.0, s1=0.0, axis=(0.0, 0.0, 1.0)):
+def _prop(kind: int = PROP_UNIFORM, width=None, table=None, n=(1, 1, 1), s=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 1.0)):
     p = _Prop()
     p.kind = kind
     keep = []
@@ -145,7 +145,8 @@ def _prop(kind: int = PROP_UNIFORM, width=None, table=None, n0=0, n1=0, s0=0.0, 
         t = np.ascontiguousarray(np.asarray(table, np.float32).reshape(-1))
         keep.append(t)
         p.width_table = _ptr(t)
-        p.n0, p.n1, p.s0, p.s1 = int(n0), int(n1), float(s0), float(s1)
+        for d in range(3):
+            p.n[d], p.s[d] = int(n[d]), float(s[d])
     for k, v in enumerate(axis):
         p.axis[k] = float(v)
     return p, keep
@@ -191,9 +192,9 @@ def cone(width) -> tuple:
     return _prop(PROP_CONE, width=width)
 
 
-def learned(table, n0, n1, s0, s1, axis) -> tuple:
-    """Learned-width descriptor: width = table[rep, i0, i1] (see oracle.c, cone_width)."""
-    return _prop(PROP_CONE, table=table, n0=n0, n1=n1, s0=s0, s1=s1, axis=axis)
+def learned(table, n, s, axis) -> tuple:
+    """Learned-width descriptor: width = table[i0, i1, i2] over quantised features (oracle.c, cone_width)."""
+    return _prop(PROP_CONE, table=table, n=n, s=s, axis=axis)
 
 
 # ------------------------------------------------------------------------------------------ bitwise

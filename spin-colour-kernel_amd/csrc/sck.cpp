@@ -17,6 +17,13 @@
 #include <vector>
 #include <sys/stat.h>
 
+// static kernels of the learned-proposal training (mlp.hip)
+extern "C" __global__ void sck_mlp_fwdbwd_k(const float *, const sck_mlp_sample *, int32_t, sck_mlp_config, float *);
+extern "C" __global__ void sck_mlp_reduce_k(const float *, int32_t, int32_t, float *);
+extern "C" __global__ void sck_mlp_adam_k(float *, float *, float *, float *, const float *, int32_t,
+                                        sck_mlp_adam_config);
+extern "C" __global__ void sck_mlp_table_k(const float *, sck_mlp_config, sck_width_grid, float *);
+
 namespace {
 
 thread_local std::string g_err;
@@ -141,6 +148,7 @@ struct sck_handle {
     sck_launch bump{};
     bool has_resident = false;
     sck_launch resident{};
+    int sample_kernel = -1;
     int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
 };
 
@@ -235,6 +243,7 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
         }
         h->fn.push_back(f);
         h->names.push_back(d->kernel_names[i]);
+        if (std::strcmp(d->kernel_names[i], "sck_sample") == 0) h->sample_kernel = i;
     }
     h->sweep.assign(d->sweep, d->sweep + d->n_sweep);
     h->ref.assign(d->ref, d->ref + d->n_ref);
@@ -350,6 +359,70 @@ int sck_graph_destroy(sck_graph *g) {
     if (g->graph) hipGraphDestroy(g->graph);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
+    return SCK_OK;
+}
+
+// ----------------------------------------------------------------------------- learned proposal width
+static const int kMlpH = 32;
+static const int kMlpParams = kMlpH * 3 + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1;
+static const int kMlpChunk = 256;
+
+int sck_mlp_nparam(void) { return kMlpParams; }
+
+int64_t sck_mlp_workspace_bytes(int32_t nsample, int32_t ndraw) {
+    int64_t nrec = (int64_t)nsample * ndraw;
+    int64_t nchunk = (nrec + kMlpChunk - 1) / kMlpChunk;
+    return 4 * (nrec * (5 + 4 * kMlpH) + nchunk * (kMlpParams + 1));
+}
+
+int sck_mlp_train_step(float *params, float *adam_m, float *adam_v, const sck_mlp_sample *samples, int32_t nsample,
+                       const sck_mlp_config *cfg, const sck_mlp_adam_config *adam, void *workspace, float *grad_out,
+                       void *stream) {
+    if (!params || !samples || nsample <= 0 || !cfg || cfg->ndraw <= 0 || !adam || !workspace || !grad_out)
+        return fail(SCK_E_INVALID, "invalid training-step arguments");
+    if (adam->lr != 0.0f && (!adam_m || !adam_v || adam->step < 1))
+        return fail(SCK_E_INVALID, "an Adam step needs its moments and a 1-based step");
+    hipStream_t s = (hipStream_t)stream;
+    int32_t nrec = nsample * cfg->ndraw;
+    float *rec = (float *)workspace;
+    int32_t nchunk = (nrec + kMlpChunk - 1) / kMlpChunk;
+    float *partial = rec + (int64_t)nrec * (5 + 4 * kMlpH);
+    sck_mlp_config c = *cfg;
+    hipLaunchKernelGGL(sck_mlp_fwdbwd_k, dim3((nrec + 127) / 128), dim3(128), 0, s, params, samples, nsample, c, rec);
+    HIP_TRY(hipGetLastError());
+    int32_t chunk = kMlpChunk;
+    hipLaunchKernelGGL(sck_mlp_reduce_k, dim3(nchunk, (kMlpParams + 1 + 1023) / 1024), dim3(1024), 0, s, rec, nrec,
+                       chunk, partial);
+    HIP_TRY(hipGetLastError());
+    sck_mlp_adam_config ac = *adam;
+    hipLaunchKernelGGL(sck_mlp_adam_k, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, s, params, adam_m, adam_v,
+                       grad_out, partial, nchunk, ac);
+    HIP_TRY(hipGetLastError());
+    return SCK_OK;
+}
+
+int sck_mlp_table(const float *params, const sck_mlp_config *cfg, const sck_width_grid *grid, float *table,
+                  void *stream) {
+    if (!params || !cfg || !grid || !table) return fail(SCK_E_INVALID, "invalid table arguments");
+    int total = grid->n[0] * grid->n[1] * grid->n[2];
+    if (total <= 0) return fail(SCK_E_INVALID, "empty width grid");
+    hipLaunchKernelGGL(sck_mlp_table_k, dim3((total + 127) / 128), dim3(128), 0, (hipStream_t)stream, params, *cfg,
+                       *grid, table);
+    HIP_TRY(hipGetLastError());
+    return SCK_OK;
+}
+
+int sck_sample_sites(sck_handle *h, const sck_args *args, const int32_t *rep_site, int32_t nsample,
+                     sck_mlp_sample *out, void *stream) {
+    if (!h || !args || !rep_site || nsample <= 0 || !out) return fail(SCK_E_INVALID, "invalid sampling arguments");
+    if (h->sample_kernel < 0) return fail(SCK_E_INVALID, "this module has no sampling kernel (vector models only)");
+    sck_args a = *args;
+    const int32_t *rs = rep_site;
+    int32_t n = nsample;
+    sck_mlp_sample *o = out;
+    void *params[] = {&a, &rs, &n, &o};
+    HIP_TRY(hipModuleLaunchKernel(h->fn[h->sample_kernel], (nsample + 127) / 128, 1, 1, 128, 1, 1, 0,
+                                  (hipStream_t)stream, params, nullptr));
     return SCK_OK;
 }
 

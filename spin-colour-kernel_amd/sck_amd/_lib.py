@@ -19,7 +19,7 @@ ROOT = PKG.parent                      # spin-colour-kernel_amd/
 REPO = ROOT.parent
 INCLUDE = REPO / "include"
 LIB = PKG / "libsck.so"
-SOURCES = [ROOT / "csrc" / "sck.cpp"]
+SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
 SCK_ABI_VERSION = 1
@@ -52,13 +52,36 @@ class Desc(C.Structure):
 #: every symbol include/sck.h declares
 EXPORTS = ["sck_version", "sck_last_error", "sck_create", "sck_precompile", "sck_destroy", "sck_sweep",
            "sck_reference_sweep", "sck_sync_sweep", "sck_observe", "sck_graph_capture", "sck_graph_replay",
-           "sck_graph_destroy", "sck_time_replays"]
+           "sck_graph_destroy", "sck_time_replays", "sck_mlp_nparam", "sck_mlp_workspace_bytes",
+           "sck_mlp_train_step", "sck_mlp_table", "sck_sample_sites"]
+
+
+class MlpSample(C.Structure):
+    _fields_ = [("s", C.c_float * 3), ("v", C.c_float * 3), ("beta", C.c_float), ("k1", C.c_float),
+                ("ku", C.c_float), ("axis", C.c_float * 3)]
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [("n", C.c_int32), ("ndraw", C.c_int32), ("seed", C.c_int32), ("wmin", C.c_float),
+                ("wmax", C.c_float), ("k2", C.c_float), ("du", C.c_float), ("axis", C.c_float * 3),
+                ("xs", C.c_float * 3)]
+
+
+class AdamConfig(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("step", C.c_int32)]
+
+
+class WidthGrid(C.Structure):
+    _fields_ = [("n", C.c_int32 * 3), ("s", C.c_float * 3)]
 
 
 def build(verbose: bool = False) -> Path:
-    """Compile libsck.so in-tree with hipcc (host code; device code is compiled per system by hipRTC)."""
-    cmd = ["hipcc", "-shared", "-fPIC", "-O2", "-std=c++17", f"-I{INCLUDE}", *map(str, SOURCES),
-           "-o", str(LIB) + ".tmp", "-lhiprtc"]
+    """Compile libsck.so in-tree with hipcc for gfx950: the C-ABI launcher and the static kernels of the
+    learned-proposal training (the sweep kernels are generated per system and compiled by hipRTC)."""
+    cmd = ["hipcc", "-shared", "-fPIC", "-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", f"-I{INCLUDE}", "-x", "hip",
+           *map(str, SOURCES), "-o", str(LIB) + ".tmp", "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"building libsck.so failed:\n{r.stdout}\n{r.stderr}")
@@ -93,6 +116,14 @@ def lib():
         L.sck_graph_replay.argtypes = [C.c_void_p, C.c_void_p]
         L.sck_graph_destroy.argtypes = [C.c_void_p]
         L.sck_time_replays.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_float)]
+        L.sck_mlp_nparam.restype = C.c_int
+        L.sck_mlp_workspace_bytes.restype = C.c_int64
+        L.sck_mlp_workspace_bytes.argtypes = [C.c_int32, C.c_int32]
+        L.sck_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                         C.POINTER(MlpConfig), C.POINTER(AdamConfig), C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+        L.sck_mlp_table.argtypes = [C.c_void_p, C.POINTER(MlpConfig), C.POINTER(WidthGrid), C.c_void_p, C.c_void_p]
+        L.sck_sample_sites.argtypes = [C.c_void_p, C.POINTER(Args), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         if L.sck_version() != SCK_ABI_VERSION:
             raise RuntimeError("libsck.so ABI version mismatch: rebuild it")
         _lib = L

@@ -427,40 +427,7 @@ class _Gen:
         # vector models: fp32 local field in the reference's order
         for k in range(n):
             w(f"    float s{k} = (float){self.at('src', k, 'own')};")
-        for k in range(n):
-            w(f"    float v{k} = 0.0f;")
-        for j in range(len(extras)):
-            for k in range(n):
-                w(f"    float n{j}_{k} = 0.0f;")
-        for Jv, members in groups:
-            w("    {")
-            for k in range(n):
-                w(f"        float g{k} = 0.0f;")
-            for i in members:
-                q, ok = self.nb_index(i)
-                w(f"        {{ const int q = {q}; const bool ok = {ok};")
-                for k in range(n):
-                    w(f"          const float t{k} = ok ? (float){self.at('src', k, 'q')} : 0.0f; g{k} = g{k} + t{k};")
-                    if i in extras:
-                        w(f"          n{extras.index(i)}_{k} = t{k};")
-                w("        }")
-            if Jv != 0.0:
-                for k in range(n):
-                    w(f"        v{k} = v{k} + {_hexf(Jv)} * g{k};")
-            w("    }")
-        for j, i in enumerate(extras):
-            E = np.asarray(s.extra[i], np.float64)
-            for k in range(n):
-                terms = [f"{_hexf(E[k, l])} * n{j}_{l}" for l in range(n) if E[k, l] != 0.0]
-                if terms:
-                    acc = terms[0]
-                    for t in terms[1:]:
-                        acc = f"({acc}) + {t}"
-                    w(f"    v{k} = v{k} + ({acc});")
-        if s.h is not None:
-            for k in range(n):
-                if s.h[k] != 0.0:
-                    w(f"    v{k} = v{k} + {_hexf(s.h[k])};")
+        self.emit_local_field(a)
         self.emit_draw("    ")
         self.emit_proposal()
         dsum = f"(t0 - s0) * v0"
@@ -498,6 +465,44 @@ class _Gen:
         w("    }")
         w("    return acc;")
         w("}")
+
+    def emit_local_field(self, a: int, ind: str = "    ") -> None:
+        """v0..v{n-1}: the local field of site (x, y, z) of sublattice a read from `src` at replica offset
+        `rbase`, in the reference's fp32 order: J groups in first-appearance order (v = v + J g), then the
+        extra matrices of the extra bonds in bond order, then the field (SCK/generator.py:222-277)."""
+        s, n, w = self.s, self.n, self.w
+        groups, extras = _groups(s, a)
+        for k in range(n):
+            w(f"{ind}float v{k} = 0.0f;")
+        for j in range(len(extras)):
+            for k in range(n):
+                w(f"{ind}float n{j}_{k} = 0.0f;")
+        for Jv, members in groups:
+            w(f"{ind}{{")
+            for k in range(n):
+                w(f"{ind}    float g{k} = 0.0f;")
+            for i in members:
+                q, ok = self.nb_index(i)
+                w(f"{ind}    {{ const int q = {q}; const bool ok = {ok};")
+                for k in range(n):
+                    w(f"{ind}      const float t{k} = ok ? (float){self.at('src', k, 'q')} : 0.0f; g{k} = g{k} + t{k};")
+                    if i in extras:
+                        w(f"{ind}      n{extras.index(i)}_{k} = t{k};")
+                w(f"{ind}    }}")
+            if Jv != 0.0:
+                for k in range(n):
+                    w(f"{ind}    v{k} = v{k} + {_hexf(Jv)} * g{k};")
+            w(f"{ind}}}")
+        for j, i in enumerate(extras):
+            E = np.asarray(s.extra[i], np.float64)
+            for k in range(n):
+                terms = [f"{_hexf(E[k, l])} * n{j}_{l}" for l in range(n) if E[k, l] != 0.0]
+                if terms:
+                    w(f"{ind}v{k} = v{k} + ({self._lassoc(terms)});")
+        if s.h is not None:
+            for k in range(n):
+                if s.h[k] != 0.0:
+                    w(f"{ind}v{k} = v{k} + {_hexf(s.h[k])};")
 
     @staticmethod
     def _lassoc(terms):
@@ -538,15 +543,23 @@ class _Gen:
         if self.proposal == "cone":
             w("    const float wd = A.width[rep];")
         else:
+            # learned width: the frozen network tabulated on quantised features of the site's environment
             lm = self.lm
             vv = self._lassoc([f"v{k} * v{k}" for k in range(n)])
             w(f"    const float hv = sqrtf({vv});")
-            w("    const float f0 = hv * A.beta[rep];")
+            w("    const float bt = A.beta[rep];")
+            kexpr = _hexf(abs(self.s.uniaxial[0]) if self.s.uniaxial is not None else 0.0)
+            if self.s.site_ku is not None:
+                kexpr = f"({kexpr} + fabsf(A.site_ku[own]))"
+            if self.s.cubic_k1 is not None:
+                kexpr = f"({kexpr} + fabsf(A.k1[own]))"
+            w(f"    const float f0 = hv * bt;")
+            w(f"    const float f1 = {kexpr} * bt;")
             ve = self._lassoc([f"v{k} * {_hexf(lm['axis'][k])}" for k in range(n)])
             w(f"    const float f2 = hv > 0.0f ? fabsf({ve}) / hv : 0.0f;")
-            w(f"    int i0 = (int)(f0 * {_hexf(lm['s0'])}); i0 = i0 > {lm['n0'] - 1} ? {lm['n0'] - 1} : (i0 < 0 ? 0 : i0);")
-            w(f"    int i1 = (int)(f2 * {_hexf(lm['s1'])}); i1 = i1 > {lm['n1'] - 1} ? {lm['n1'] - 1} : (i1 < 0 ? 0 : i1);")
-            w(f"    const float wd = A.width[((int64_t)rep * {lm['n0']} + i0) * {lm['n1']} + i1];")
+            for d in range(3):
+                w(f"    int i{d} = (int)(f{d} * {_hexf(lm['s'][d])}); i{d} = i{d} > {lm['n'][d] - 1} ? {lm['n'][d] - 1} : (i{d} < 0 ? 0 : i{d});")
+            w(f"    const float wd = A.width[(i0 * {lm['n'][1]} + i1) * {lm['n'][2]} + i2];")
         if n == 2:
             w("    const float ph = atan2f(s1, s0) + wd * (2.0f * u1 - 1.0f);")
             w("    float t0, t1; sincos_bounded(ph, &t1, &t0);")
@@ -616,6 +629,8 @@ class _Gen:
             w("    }")
         w("}")
         w()
+        if self.n > 1:
+            self.emit_sample_kernel(T)
         # synchronous control: every site of sublattice a from the old state
         for a in range(self.b):
             w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_sync_{a}(sck_args A) {{')
@@ -675,6 +690,41 @@ class _Gen:
         w("        const int rep = rep0 + 64 * i;")
         w(f"        if (rep < {self.B}) upd_{a}(A, S, S, (int64_t)rep, rep, own, x, y, z, key);")
         w("    }")
+        w("}")
+        w()
+
+    def emit_sample_kernel(self, T: str) -> None:
+        """sck_sample: training records for the learned proposal, one per (replica, site) pair of rs: the
+        spin, the local field of its neighbours, beta and the site's anisotropy constants (12 floats, the
+        layout of sck_mlp_sample)."""
+        w = self.w
+        w(f'extern "C" __global__ void __launch_bounds__(128) sck_sample(sck_args A, const int32_t *rs, int32_t nsample, '
+          f'float *out) {{')
+        w("    const int i = blockIdx.x * blockDim.x + threadIdx.x;")
+        w("    if (i >= nsample) return;")
+        w("    const int rep = rs[2 * i], own = rs[2 * i + 1];")
+        w(f"    const int a = own / {self.PL}, rem = own - a * {self.PL};")
+        w(f"    const int x = rem % {self.L1}, row = rem / {self.L1};")
+        w(f"    const int z = row / {self.L2}, y = row % {self.L2};")
+        w(f"    const {T} *src = (const {T} *)A.spins;")
+        w(f"    const int64_t rbase = {self.rbase_expr('rep')};")
+        w("    float *o = out + (int64_t)i * 12;")
+        w("    for (int k = 0; k < 12; ++k) o[k] = 0.0f;")
+        for k in range(self.n):
+            w(f"    o[{k}] = (float){self.at('src', k, 'own')};")
+        for a in range(self.b):
+            w(f"    if (a == {a}) {{")
+            self.emit_local_field(a, "        ")
+            for k in range(self.n):
+                w(f"        o[{3 + k}] = v{k};")
+            w("    }")
+        w("    o[6] = A.beta[rep];")
+        if self.s.cubic_k1 is not None:
+            w("    o[7] = A.k1[own];")
+        if self.s.site_ku is not None:
+            w("    o[8] = A.site_ku[own];")
+            for k in range(self.n):
+                w(f"    o[{9 + k}] = A.site_axis[(int64_t)own * {self.n} + {k}];")
         w("}")
         w()
 
@@ -1343,7 +1393,8 @@ class _Gen:
         src = "\n".join(self.lines) + "\n"
         kernels = [f"sck_class_{k}" for k in range(len(self.classes))] + ["sck_seq"] + \
                   [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"] + \
-                  (["sck_resident"] if rplan is not None else [])
+                  (["sck_resident"] if rplan is not None else []) + \
+                  (["sck_sample"] if (self.n > 1 and self.mode == MODE_F32) else [])
         ki = {k: i for i, k in enumerate(kernels)}
         R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
         gy, gz = (R, 1) if R <= 65535 else (65535, -(-R // 65535))

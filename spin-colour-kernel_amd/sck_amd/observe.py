@@ -1,0 +1,134 @@
+"""Observable series, accumulators and error analysis on top of the FP64 reduction kernels.
+
+* `series`: energy and |M| per replica after every `every` sweeps, captured with the sweeps in one HIP graph
+  (the reference captures "sweep + observe" the same way, SCK/simulate.py:361-376).
+* `accumulate`: the reference's per-temperature accumulators about a reference energy E_ref (dE, dE^2, |M|,
+  |M|^2; SCK/simulate.py:349-398) and Cv = var(E) / (T^2 N), chi = N var(|M|) / T per replica, averaged
+  (SCK/simulate.py:521-526).
+* `integrated_tau`: Sokal's automatic window, c = 6, ACF averaged over chains (SCK/exp_core.py:157-173).
+* `jackknife`: delete-one-block jackknife error of any statistic of per-chain data (new: the reference
+  jackknifes only Tc, remote/e4_jackknife.py:63-80).
+* `effective_samples`: sum over temperatures of chains x measurements / (2 tau_int).
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+__all__ = ["series", "accumulate", "integrated_tau", "jackknife", "effective_samples", "cv_chi"]
+
+
+def series(sampler, S: torch.Tensor, n_meas: int, every: int = 1):
+    """(E (n_meas, B), |M|/N (n_meas, B), seconds) from n_meas replays of a graph of `every` sweeps followed by
+    one observation; E in float64 totals, |M| the norm of the total magnetization per active site."""
+    B = sampler.B
+    N = float(sampler.sys.n_active)
+    dev = sampler.device
+    rep = sampler.graph(S, every, observe=True)
+    nv = sampler.sys.b * sampler.sys.n
+    out = torch.zeros((n_meas, B * (1 + nv)), dtype=torch.float64, device=dev)
+    obs = sampler._obs
+    rep()                                      # warm-up replay
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for i in range(n_meas):
+        rep()
+        out[i].copy_(obs)
+    torch.cuda.synchronize(dev)
+    dt = time.perf_counter() - t0
+    E = out[:, :B]
+    M = out[:, B:].reshape(n_meas, B, sampler.sys.b, sampler.sys.n).sum(dim=2)
+    return E.cpu().numpy(), (M.norm(dim=2) / N).cpu().numpy(), dt
+
+
+def integrated_tau(x: np.ndarray, c: float = 6.0) -> float:
+    """Integrated autocorrelation time of x (time, chains): tau = 1 + 2 sum_{t <= M} rho(t), the window M the
+    first with M >= c tau; the ACF is averaged over chains via FFT; a constant series has tau = 1."""
+    x = np.asarray(x, np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+    x = x - x.mean(axis=0)
+    n = x.shape[0]
+    if not np.all(np.isfinite(x)) or float((x * x).sum()) == 0.0:
+        return 1.0
+    f = np.fft.rfft(x, n=2 * n, axis=0)
+    acf = np.fft.irfft(f * np.conj(f), axis=0)[:n].mean(axis=1)
+    acf = acf / acf[0]
+    if not np.isfinite(acf).all():
+        return 1.0
+    tau = 1.0
+    csum = np.cumsum(acf[1:])
+    for M in range(1, n):
+        tau = 1.0 + 2.0 * csum[M - 1]
+        if M >= c * tau:
+            break
+    return max(float(tau), 0.5)
+
+
+def jackknife(stat, data: np.ndarray, blocks: int = 16):
+    """(value, error) of stat(data) with delete-one-block jackknife over the last axis (chains)."""
+    data = np.asarray(data)
+    n = data.shape[-1]
+    blocks = max(2, min(blocks, n))
+    edges = np.linspace(0, n, blocks + 1).astype(int)
+    full = stat(data)
+    vals = []
+    for b in range(blocks):
+        keep = np.concatenate([np.arange(0, edges[b]), np.arange(edges[b + 1], n)])
+        vals.append(stat(data[..., keep]))
+    vals = np.asarray(vals)
+    err = np.sqrt((blocks - 1) / blocks * ((vals - vals.mean(axis=0)) ** 2).sum(axis=0))
+    return full, err
+
+
+def cv_chi(E: np.ndarray, Mabs: np.ndarray, T: float, N: float):
+    """Per-replica Cv = var(E)/(T^2 N) and chi = N var(|M|)/T from series (time, chains), averaged over chains
+    with their jackknife errors."""
+    def cv(e):
+        return float((e.var(axis=0) / (T * T * N)).mean())
+
+    def chi(m):
+        return float((N * m.var(axis=0) / T).mean())
+    return jackknife(cv, E), jackknife(chi, Mabs)
+
+
+def accumulate(sampler, S: torch.Tensor, temps, n_therm: int, n_meas: int, every: int = 1):
+    """Temperature ladder with the reference's accumulators; returns per temperature E, E2, |M|, M2 per replica
+    and Cv, chi per site averaged over replicas."""
+    temps = np.asarray(temps, float)
+    N = float(sampler.sys.n_active)
+    out = {k: np.zeros((len(temps), sampler.B)) for k in ("E", "E2", "Mabs", "M2")}
+    cv, chi = np.zeros(len(temps)), np.zeros(len(temps))
+    for t, T in enumerate(temps):
+        sampler.set_temperature(float(T))
+        sampler.sweep(S, int(n_therm))
+        E, Ma, _ = series(sampler, S, int(n_meas), every)
+        Er = E[0]
+        dE = E - Er
+        out["E"][t] = Er + dE.mean(0)
+        out["E2"][t] = (dE * dE).mean(0) + 2 * Er * dE.mean(0) + Er * Er
+        out["Mabs"][t] = Ma.mean(0)
+        out["M2"][t] = (Ma * Ma).mean(0)
+        varE = np.maximum(out["E2"][t] - out["E"][t] ** 2, 0.0)
+        varM = np.maximum(out["M2"][t] - out["Mabs"][t] ** 2, 0.0)
+        cv[t] = (varE / (T * T * N)).mean()
+        chi[t] = (N * varM / T).mean()
+    out["Cv"], out["chi"] = cv, chi
+    return out
+
+
+def effective_samples(x: np.ndarray, groups: np.ndarray):
+    """Effective independent samples of series x (time, B) whose replicas fall into temperature groups:
+    sum over groups of chains * time / (2 tau_int of the group), and the per-group tau."""
+    groups = np.asarray(groups)
+    taus = {}
+    ess = 0.0
+    for g in np.unique(groups):
+        xs = x[:, groups == g]
+        tau = integrated_tau(xs)
+        taus[int(g)] = tau
+        ess += xs.shape[1] * xs.shape[0] / (2.0 * tau)
+    return ess, taus

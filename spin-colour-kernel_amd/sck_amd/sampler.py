@@ -180,7 +180,9 @@ class Sampler:
                 np.asarray(width, np.float32).reshape(-1), (self.Bp,))
             self.width = torch.as_tensor(np.ascontiguousarray(w), device=dev)
         elif proposal == "learned":
-            self.width = torch.as_tensor(np.ascontiguousarray(learned.table(self._beta_np)), device=dev)
+            if learned is None:
+                raise ValueError("proposal='learned' needs a trained LearnedWidth (sck_amd.proposal.train_width)")
+            self.width = torch.as_tensor(np.ascontiguousarray(learned.table()), device=dev)
             self._learned = learned
         else:
             self.width = torch.zeros(1, dtype=torch.float32, device=dev)
@@ -311,8 +313,6 @@ class Sampler:
             self.ptab.copy_(torch.as_tensor(self._table(self.sys, bwd[:, 0].astype(np.float64)), device=self.device))
         elif self.sys.n == 1:
             self.table.copy_(torch.as_tensor(G.ising_table(self.sys, beta), device=self.device))
-        if self.proposal == "learned":
-            self.width.copy_(torch.as_tensor(self._learned.table(beta), device=self.device))
         self._beta_np = beta
         self.beta.copy_(torch.as_tensor(beta, device=self.device))
 
