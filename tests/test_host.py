@@ -108,7 +108,7 @@ def test_generator_modes_emit_all_kernels():
                     (sy.ising("sc", 4), hg.MODE_BITWISE), (sy.ising("sc", 4), hg.MODE_BITWISE_INDEP)]:
         m = hg.generate(s, mode, 64)
         for k in m.kernels:
-            assert re.search(rf"__global__ void .*\b{k}\(sck_args A\)", m.src), k
+            assert re.search(rf"__global__ void .*\b{k}\(sck_args A[,)]", m.src), k
         assert len(m.sweep) == len(s.classes())
 
 
