@@ -115,7 +115,7 @@ def test_generator_modes_emit_all_kernels():
 def test_library_exports_every_declared_symbol():
     from sck_amd import _lib as L
     header = (REPO / "include" / "sck.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(sck_\w+)\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(sck_\w+)\(", header, re.M))
     assert declared == set(L.EXPORTS)
     lib = ctypes.CDLL(str(L.LIB))
     for name in declared:
