@@ -119,13 +119,16 @@ def learned_main(a) -> None:
     s, _, B, T, _ = _make("heisenberg")
     temps = np.unique(T.numpy())
     meas = max(50, a.steps)
-    r = compare_proposals(s, B, temps, warmup=200, therm=max(20, a.warmup), n_meas=meas, log=lambda m: print(m, flush=True))
+    r = compare_proposals(s, B, temps, warmup=200, therm=max(20, a.warmup), n_meas=meas,
+                          log=lambda m: print(m, flush=True))
     out = {"metric": "effective samples/s, learned vs fixed proposal", "value": r["gain_E"],
-           "unit": "ratio of ESS/s (energy)", "n_gpus": 1, "steps": meas, "warmup": a.warmup,
+           "unit": "ratio of ESS/s of the energy, geometric mean over the temperature ladder", "n_gpus": 1, "steps": meas, "warmup": a.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (random initial spins from the seeded torch CPU generator)",
            "config": {"workload": WORKLOADS["learned"], "batch": B, "sites": s.n_active},
-           "gain_E": r["gain_E"], "gain_M": r["gain_M"], "fixed_width": r["fixed_width"],
+           "gain_E": r["gain_E"], "gain_M": r["gain_M"], "gain_E_bottleneck": r["gain_E_bottleneck"],
+           "gain_M_bottleneck": r["gain_M_bottleneck"], "gain_E_sum": r["gain_E_sum"], "gain_M_sum": r["gain_M_sum"],
+           "fixed_width": r["fixed_width"],
            "ess_per_s": {k: {"E": r[k]["ess_E_per_s"], "M": r[k]["ess_M_per_s"],
                              "updates_per_s": r[k]["updates_per_s"]} for k in ("fixed", "learned")},
            "observables_max_z": r["max_z"], "observables_frac_within_2sigma": r["frac_within_2sigma"],

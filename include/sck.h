@@ -165,9 +165,11 @@ int sck_graph_destroy(sck_graph *g);
 int sck_time_replays(sck_graph *g, int32_t reps, void *stream, float *ms);
 
 /* Learned proposal: number of network parameters (3 -> 32 -> 32 -> 1), workspace bytes of one training
- * step, one training step (forward + backward over nsample x ndraw draws, fixed-order gradient sums, Adam;
- * grad_out receives the gradient of -mean objective and, last, the summed objective), and the width table of
- * the frozen network on a feature grid. No counterpart in the reference. */
+ * step, one training step (forward + backward over nsample samples of ndraw cone draws each, fixed-order
+ * gradient sums, Adam; grad_out receives the gradient of L = -mean log(1e-4 + J_i), J_i a sample's expected
+ * squared jump, and, last, the sum of log(1e-4 + J_i)), and the width table of the frozen network on a feature
+ * grid.
+ * No counterpart in the reference. */
 int sck_mlp_nparam(void);
 int64_t sck_mlp_workspace_bytes(int32_t nsample, int32_t ndraw);
 int sck_mlp_train_step(float *params, float *adam_m, float *adam_v, const sck_mlp_sample *samples, int32_t nsample,

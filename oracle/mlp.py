@@ -2,8 +2,8 @@
 
 Restates spin-colour-kernel_amd/csrc/mlp.hip in numpy: the features of a training sample, the 3 -> 32 -> 32 -> 1 tanh
 network, the cone proposal of half-angle w = wmin + (wmax - wmin) sigmoid(o) with draws (u1, u2) from Philox
-(key = seed, counter = sample * ndraw + draw), the objective J = min(1, exp(-beta dE)) |t - s|^2 and the gradient
-of L = -mean J with respect to the parameters by backpropagation. The reference has no learned proposal, so this
+(key = seed, counter = sample * ndraw + draw), a sample's expected squared jump J_i = mean over its draws of
+min(1, exp(-beta dE)) |t - s|^2, and the gradient of L = -mean_i log(1e-4 + J_i) by backpropagation. The reference has no learned proposal, so this
 part is pinned by mathematics (finite differences of the same objective) rather than by reference vectors.
 """
 
@@ -115,8 +115,12 @@ def delta_e(S, t, cfg):
     return dE, g
 
 
+EPS_J = 1e-4
+
+
 def objective_and_grad(P, samples, cfg):
-    """(mean J, dL/dP with L = -mean J) in float64, over nsample x ndraw draws."""
+    """(mean_i log(EPS_J + J_i), dL/dP) in float64 with J_i a sample's expected squared jump over its ndraw
+    draws and L = -mean_i log(EPS_J + J_i)."""
     S = np.asarray(samples, np.float64)
     N, K, n = S.shape[0], cfg["ndraw"], cfg["n"]
     x = features(S, cfg)
@@ -125,8 +129,8 @@ def objective_and_grad(P, samples, cfg):
     w = cfg["wmin"] + (cfg["wmax"] - cfg["wmin"]) * sg
     dwdo = (cfg["wmax"] - cfg["wmin"]) * sg * (1 - sg)
     u1, u2 = draws(N, K, cfg["seed"])
-    Jsum = 0.0
-    dLdo = np.zeros(N)
+    Ji = np.zeros(N)
+    dJdw = np.zeros(N)
     for k in range(K):
         t, dt = cone(S[:, :n], w, u1[:, k], u2[:, k], n)
         dE, g = delta_e(S, t, cfg)
@@ -134,16 +138,29 @@ def objective_and_grad(P, samples, cfg):
         a = np.where(xb >= 0, 1.0, np.exp(np.minimum(xb, 0.0)))
         dadE = np.where(xb >= 0, 0.0, -S[:, 6] * a)
         d = ((t - S[:, :n]) ** 2).sum(1)
-        Jsum += (a * d).sum()
-        dJdw = ((d * dadE)[:, None] * g * dt + (a[:, None] * 2 * (t - S[:, :n])) * dt).sum(1)
-        dLdo += -dJdw * dwdo / (N * K)
+        Ji += a * d / K
+        dJdw += ((d * dadE)[:, None] * g * dt + (a[:, None] * 2 * (t - S[:, :n])) * dt).sum(1) / K
+    dLdo = -dJdw / (EPS_J + Ji) * dwdo / N
+    obj = float(np.log(EPS_J + Ji).mean())
     W1, b1, W2, b2, W3, b3 = unpack_params(P)
     d2 = dLdo[:, None] * W3[None, :] * (1 - h2 * h2)
     d1 = (d2 @ W2) * (1 - h1 * h1)
     grad = np.concatenate([(d1.T @ x).reshape(-1), d1.sum(0), (d2.T @ h1).reshape(-1), d2.sum(0),
                            dLdo @ h2, [dLdo.sum()]])
-    return Jsum / (N * K), grad
+    return obj, grad
 
 
 def loss(P, samples, cfg) -> float:
-    return -objective_and_grad(P, samples, cfg)[0]
+    """L = -mean_i log(EPS_J + J_i), the quantity objective_and_grad differentiates."""
+    S = np.asarray(samples, np.float64)
+    N, K, n = S.shape[0], cfg["ndraw"], cfg["n"]
+    o, _, _ = forward(P, features(S, cfg))
+    w = cfg["wmin"] + (cfg["wmax"] - cfg["wmin"]) / (1 + np.exp(-o))
+    u1, u2 = draws(N, K, cfg["seed"])
+    Ji = np.zeros(N)
+    for k in range(K):
+        t, _ = cone(S[:, :n], w, u1[:, k], u2[:, k], n)
+        dE, _ = delta_e(S, t, cfg)
+        a = np.exp(np.minimum(-S[:, 6] * dE, 0.0))
+        Ji += a * ((t - S[:, :n]) ** 2).sum(1) / K
+    return float(-np.log(EPS_J + Ji).mean())

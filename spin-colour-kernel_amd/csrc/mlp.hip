@@ -7,10 +7,10 @@
 // to the half-angle w = WMIN + (pi - WMIN) sigmoid(o) of a cone proposal around the current spin. Because w is
 // a function of those features only, the reverse move sees the same w and the proposal stays symmetric.
 //
-// Objective: the expected squared jump distance per update, E_t[min(1, exp(-beta dE(s -> t))) |t - s|^2], for
+// Objective: the expected squared jump distance per update, J = E_t[min(1, exp(-beta dE(s -> t))) |t - s|^2], for
 // trial spins t = T(s, w, u1, u2) of the cone proposal, estimated with K draws (u1, u2) per sample, so the
-// gradient flows through w by reparameterisation. Training maximises its mean over samples taken from the
-// sampler's own warm-up sweeps (sck_amd.proposal).
+// gradient flows through w by reparameterisation. Training maximises the mean of log(1e-4 + J) over samples
+// taken from the sampler's own warm-up sweeps (sck_amd.proposal).
 //
 // Kernels: sck_mlp_fwdbwd_k (per sample and draw: forward, objective, backward to the layer deltas),
 // sck_mlp_reduce_k (per-parameter partial sums over sample chunks, fixed order), sck_mlp_adam_k (sum the partials,
@@ -152,18 +152,19 @@ __device__ void cone(const sck_mlp_config &cfg, const float *s, float w, float u
 
 }  // namespace
 
-// Per (sample, draw): features -> network -> w -> trial spin -> objective J = a |t - s|^2 and dJ/do; then the
-// layer deltas of the backward pass. Writes per-(sample, draw) rows of the backward record
+// Per sample: features -> network -> w, then ndraw cone draws give the sample's expected squared jump
+// J_i = mean_k min(1, exp(-beta dE)) |t_k - s|^2 and dJ_i/dw. The loss is L = -mean_i log(EPS_J + J_i): a
+// sample's weight does not depend on the scale of its jumps, so cold and hot temperatures count alike. The
+// backward pass writes one record per sample:
 //   rec[0..2] = x, rec[3..3+H) = h1, rec[3+H..3+2H) = h2, rec[3+2H..3+3H) = delta1, rec[3+3H..3+4H) = delta2,
-//   rec[3+4H] = delta3 (= dL/do), rec[4+4H] = J,
-// with L = -J / (nsample * ndraw) (the gradient of the mean objective, minimised).
+//   rec[3+4H] = delta3 (= dL/do), rec[4+4H] = J_i.
+#define EPS_J 1e-4f
 extern "C" __global__ void __launch_bounds__(128) sck_mlp_fwdbwd_k(const float *__restrict__ P,
                                                                  const sck_mlp_sample *__restrict__ samples,
                                                                  int32_t nsample, sck_mlp_config cfg,
                                                                  float *__restrict__ rec) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nsample * cfg.ndraw) return;
-    const int i = id / cfg.ndraw, k = id - i * cfg.ndraw;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsample) return;
     const sck_mlp_sample S = samples[i];
     float x[NIN];
     const int n = cfg.n;
@@ -176,23 +177,26 @@ extern "C" __global__ void __launch_bounds__(128) sck_mlp_fwdbwd_k(const float *
     const float sg = sigmoidf_(o);
     const float w = cfg.wmin + (cfg.wmax - cfg.wmin) * sg;
     const float dwdo = (cfg.wmax - cfg.wmin) * sg * (1.0f - sg);
-    uint32_t c0 = (uint32_t)id, c1 = 0u, c2 = 0u, c3 = 0u;
-    philox10(c0, c1, c2, c3, (uint32_t)cfg.seed, 0u);
-    const float u1 = u01(c0), u2 = u01(c1);
-    float t[3] = {0, 0, 0}, dt[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-    cone(cfg, S.s, w, u1, u2, t, dt);
-    const float dE = delta_e(S, cfg, t, g);
-    const float xb = -S.beta * dE;
-    const float a = xb >= 0.0f ? 1.0f : expf(xb);
-    const float dadE = xb >= 0.0f ? 0.0f : -S.beta * a;
-    float d = 0.0f;
-    for (int q = 0; q < n; ++q) d += (t[q] - S.s[q]) * (t[q] - S.s[q]);
-    const float J = a * d;
-    float dJdw = 0.0f;
-    for (int q = 0; q < n; ++q) dJdw += (d * dadE * g[q] + a * 2.0f * (t[q] - S.s[q])) * dt[q];
-    const float scale = -1.0f / (float)(nsample * cfg.ndraw);
-    const float d3 = scale * dJdw * dwdo;
-    float *r = rec + (int64_t)id * (5 + 4 * H);
+    float Ji = 0.0f, dJdw = 0.0f;
+    for (int k = 0; k < cfg.ndraw; ++k) {
+        uint32_t c0 = (uint32_t)(i * cfg.ndraw + k), c1 = 0u, c2 = 0u, c3 = 0u;
+        philox10(c0, c1, c2, c3, (uint32_t)cfg.seed, 0u);
+        const float u1 = u01(c0), u2 = u01(c1);
+        float t[3] = {0, 0, 0}, dt[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        cone(cfg, S.s, w, u1, u2, t, dt);
+        const float dE = delta_e(S, cfg, t, g);
+        const float xb = -S.beta * dE;
+        const float a = xb >= 0.0f ? 1.0f : expf(xb);
+        const float dadE = xb >= 0.0f ? 0.0f : -S.beta * a;
+        float d = 0.0f;
+        for (int q = 0; q < n; ++q) d += (t[q] - S.s[q]) * (t[q] - S.s[q]);
+        Ji += a * d;
+        for (int q = 0; q < n; ++q) dJdw += (d * dadE * g[q] + a * 2.0f * (t[q] - S.s[q])) * dt[q];
+    }
+    Ji /= (float)cfg.ndraw;
+    dJdw /= (float)cfg.ndraw;
+    const float d3 = -dJdw / (EPS_J + Ji) * dwdo / (float)nsample;
+    float *r = rec + (int64_t)i * (5 + 4 * H);
     for (int q = 0; q < NIN; ++q) r[q] = x[q];
     for (int j = 0; j < H; ++j) {
         r[3 + j] = h1[j];
@@ -209,7 +213,7 @@ extern "C" __global__ void __launch_bounds__(128) sck_mlp_fwdbwd_k(const float *
         r[3 + 2 * H + j] = sacc * (1.0f - h1[j] * h1[j]);
     }
     r[3 + 4 * H] = d3;
-    r[4 + 4 * H] = J;
+    r[4 + 4 * H] = Ji;
 }
 
 // Partial gradient sums: blockIdx.x = chunk of records, one thread per parameter (NPARAM + 1 threads; the
@@ -238,8 +242,8 @@ extern "C" __global__ void __launch_bounds__(1024) sck_mlp_reduce_k(const float 
             gv = r[3 + 4 * H] * r[3 + H + (p - OFF_W3)];
         } else if (p == OFF_B3) {               // b3
             gv = r[3 + 4 * H];
-        } else {                                // the objective J (for monitoring)
-            gv = r[4 + 4 * H];
+        } else {                                // the objective log(EPS_J + J_i) (for monitoring)
+            gv = logf(EPS_J + r[4 + 4 * H]);
         }
         acc += gv;
     }

@@ -370,7 +370,8 @@ static const int kMlpChunk = 256;
 int sck_mlp_nparam(void) { return kMlpParams; }
 
 int64_t sck_mlp_workspace_bytes(int32_t nsample, int32_t ndraw) {
-    int64_t nrec = (int64_t)nsample * ndraw;
+    (void)ndraw;
+    int64_t nrec = (int64_t)nsample;
     int64_t nchunk = (nrec + kMlpChunk - 1) / kMlpChunk;
     return 4 * (nrec * (5 + 4 * kMlpH) + nchunk * (kMlpParams + 1));
 }
@@ -383,7 +384,7 @@ int sck_mlp_train_step(float *params, float *adam_m, float *adam_v, const sck_ml
     if (adam->lr != 0.0f && (!adam_m || !adam_v || adam->step < 1))
         return fail(SCK_E_INVALID, "an Adam step needs its moments and a 1-based step");
     hipStream_t s = (hipStream_t)stream;
-    int32_t nrec = nsample * cfg->ndraw;
+    int32_t nrec = nsample;
     float *rec = (float *)workspace;
     int32_t nchunk = (nrec + kMlpChunk - 1) / kMlpChunk;
     float *partial = rec + (int64_t)nrec * (5 + 4 * kMlpH);

@@ -1288,12 +1288,22 @@ class _Gen:
         w("}")
         w()
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_obs(sck_args A) {{')
-        w("    const int rep = blockIdx.y;")
-        w(f"    double acc[{NV}];")
-        w(f"    for (int k = 0; k < {NV}; ++k) acc[k] = 0.0;")
-        w(f"    const int lo = blockIdx.x * {chunk};")
-        w(f"    const int hi = min(lo + {chunk}, {self.NS});")
-        w("    for (int site = lo + threadIdx.x; site < hi; site += blockDim.x) {")
+        if self.replica_major:
+            # lanes over replicas: each thread sums its replica over the chunk's sites (uniform site addresses)
+            w("    const int rep = blockIdx.y * blockDim.x + threadIdx.x;")
+            w(f"    if (rep >= {self.B}) return;")
+            w(f"    double acc[{NV}];")
+            w(f"    for (int k = 0; k < {NV}; ++k) acc[k] = 0.0;")
+            w(f"    const int lo = blockIdx.x * {chunk};")
+            w(f"    const int hi = min(lo + {chunk}, {self.NS});")
+            w("    for (int site = lo; site < hi; ++site) {")
+        else:
+            w("    const int rep = blockIdx.y;")
+            w(f"    double acc[{NV}];")
+            w(f"    for (int k = 0; k < {NV}; ++k) acc[k] = 0.0;")
+            w(f"    const int lo = blockIdx.x * {chunk};")
+            w(f"    const int hi = min(lo + {chunk}, {self.NS});")
+            w("    for (int site = lo + threadIdx.x; site < hi; site += blockDim.x) {")
         w(f"        const int a = site / {self.PL}, rem = site - a * {self.PL};")
         w(f"        const int x = rem % {self.L1}, row = rem / {self.L1};")
         w(f"        const int z = row / {self.L2}, y = row % {self.L2};")
@@ -1336,6 +1346,15 @@ class _Gen:
             w(f"        {{ const double sn = {sn}; e -= (double)A.site_ku[site] * sn * sn; }}")
         w("        acc[0] += e;")
         w("    }")
+        if self.replica_major:
+            w(f"    for (int k = 0; k < {NV}; ++k) A.obs_scratch[((int64_t)rep * gridDim.x + blockIdx.x) * {NV} + k] = acc[k];")
+            w("}")
+        else:
+            self.emit_obs_block_reduce(NV)
+        self.emit_obs_final(NV)
+
+    def emit_obs_block_reduce(self, NV: int) -> None:
+        w = self.w
         # block reduction: wave butterfly, then LDS across the waves
         w(f"    __shared__ double red[{BLOCK // 64}][{NV}];")
         w(f"    for (int k = 0; k < {NV}; ++k) {{")
@@ -1350,6 +1369,9 @@ class _Gen:
         w(f"        A.obs_scratch[((int64_t)rep * gridDim.x + blockIdx.x) * {NV} + threadIdx.x] = v;")
         w("    }")
         w("}")
+
+    def emit_obs_final(self, NV: int) -> None:
+        w = self.w
         w()
         w(f'extern "C" __global__ void __launch_bounds__(64) sck_obs_final(sck_args A) {{')
         w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
@@ -1385,9 +1407,13 @@ class _Gen:
         else:
             self.emit_bitwise()
         # observables: enough blocks per replica for a full chip when replicas are few
-        per = max(1, min(64, (4096 + self.B - 1) // max(1, self.B)))
-        chunk = max(BLOCK, -(-self.NS // per))
-        chunk = -(-chunk // BLOCK) * BLOCK
+        if self.replica_major:
+            per = max(1, min(self.NS, -(-2048 // -(-self.B // BLOCK))))
+            chunk = -(-self.NS // per)
+        else:
+            per = max(1, min(64, (4096 + self.B - 1) // max(1, self.B)))
+            chunk = max(BLOCK, -(-self.NS // per))
+            chunk = -(-chunk // BLOCK) * BLOCK
         self.obs_chunks = -(-self.NS // chunk)
         self.emit_observe(chunk)
         src = "\n".join(self.lines) + "\n"
@@ -1421,7 +1447,7 @@ class _Gen:
                 sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.B // BLOCK), gy2, -(-self.PL // gy2))))
             else:
                 sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.PL // BLOCK), gy, gz)))
-        obs = [Launch(ki["sck_obs"], (self.obs_chunks, self.B, 1)),
+        obs = [Launch(ki["sck_obs"], (self.obs_chunks, -(-self.B // BLOCK) if self.replica_major else self.B, 1)),
                Launch(ki["sck_obs_final"], (-(-self.B // 64), 1, 1), (64, 1, 1))]
         bump = Launch(ki["sck_bump"], (1, 1, 1), (64, 1, 1))
         layouts, offsets, total = ising_layout(s) if s.n == 1 else ([], [], 0)

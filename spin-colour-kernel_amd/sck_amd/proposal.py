@@ -9,8 +9,9 @@ symmetric and plain Metropolis keeps detailed balance exactly. The learned width
     f0 = |v| beta,  f1 = (|Du| + |Ku_i| + |K1_i|) beta,  f2 = |v . e| / |v|
 
 (v the local field of the neighbours, e the anisotropy axis). It is trained on the GPU by libsck's own forward
-and backward kernels (csrc/mlp.hip) on records of the sampler's warm-up sweeps, maximising the expected squared
-jump distance per update, then frozen into a table over quantised features that the sweep kernel reads.
+and backward kernels (csrc/mlp.hip) on records of the sampler's warm-up sweeps, maximising the mean log of the
+expected squared jump distance per update (so cold and hot sites weigh alike), then frozen into a table over
+quantised features that the sweep kernel reads.
 """
 
 from __future__ import annotations
@@ -84,8 +85,8 @@ def collect_samples(sampler, S: torch.Tensor, count: int, seed: int = 0) -> torc
 
 def grad_step(params: torch.Tensor, samples: torch.Tensor, cfg: dict, lr: float = 0.0, step: int = 1,
               m: torch.Tensor | None = None, v: torch.Tensor | None = None, workspace: torch.Tensor | None = None):
-    """One GPU training step (lr = 0: gradient only). Returns grad (NPARAM + 1,): dL/dP with L = -mean J, and
-    the summed objective last."""
+    """One GPU training step (lr = 0: gradient only). Returns grad (NPARAM + 1,): dL/dP with
+    L = -mean_i log(1e-4 + J_i), and the summed objective log(1e-4 + J_i) last."""
     dev = params.device
     ns = int(samples.shape[0])
     need = int(L.lib().sck_mlp_workspace_bytes(ns, cfg["ndraw"]))
@@ -175,10 +176,10 @@ def train_width(sampler, S: torch.Tensor, warmup: int = 200, snapshots: int = 10
         cfg["seed"] = seed + step
         g = grad_step(P, samples[idx].contiguous(), cfg, lr=lr, step=step, m=m, v=v, workspace=ws)
         if step == 1 or step % 50 == 0 or step == steps:
-            J = float(g[-1]) / (min(batch, samples.shape[0]) * ndraw)
+            J = float(g[-1]) / min(batch, samples.shape[0])
             hist.append((step, J))
             if log:
-                log(f"  width training step {step:4d}: mean squared jump per update {J:.5f}")
+                log(f"  width training step {step:4d}: mean log expected squared jump {J:.5f}")
     torch.cuda.synchronize()
     cfg["seed"] = seed
     lw = LearnedWidth(P.cpu().numpy(), cfg, n=n, s=s, history=hist)
@@ -188,8 +189,9 @@ def train_width(sampler, S: torch.Tensor, warmup: int = 200, snapshots: int = 10
 
 
 def best_fixed_width(sampler, samples: torch.Tensor, widths=(0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.14159), ndraw=4):
-    """The single cone half-angle with the largest mean squared jump per update over the training records:
-    the fixed-width baseline the learned width is judged against (same objective, one parameter)."""
+    """The single cone half-angle with the largest training objective (mean log expected squared jump) over
+    the training records: the fixed-width baseline the learned width is judged against (the same objective
+    restricted to one constant)."""
     best, bestJ = None, -1.0
     cfg = mlp_config(sampler, ndraw=ndraw)
     dev = samples.device
@@ -201,7 +203,7 @@ def best_fixed_width(sampler, samples: torch.Tensor, widths=(0.25, 0.5, 0.75, 1.
         q = min(max(q, 1e-6), 1 - 1e-6)
         P[-1] = math.log(q / (1 - q))
         g = grad_step(torch.as_tensor(P, device=dev), samples, cfg)
-        J = float(g[-1]) / (samples.shape[0] * ndraw)
+        J = float(g[-1]) / samples.shape[0]
         res.append((w, J))
         if J > bestJ:
             best, bestJ = w, J

@@ -51,7 +51,7 @@ def test_oracle_gradient_matches_finite_differences(n):
     cfg = _cfg(n)
     P = init_params(3).astype(np.float64)
     J, g = OM.objective_and_grad(P, S, cfg)
-    assert J > 0
+    assert np.isfinite(J)
     rng = np.random.default_rng(1)
     idx = np.concatenate([rng.choice(OM.NPARAM, 12, replace=False), [OM.NPARAM - 1, OM.OFF_B1, OM.OFF_W3]])
     eps = 1e-5
@@ -93,7 +93,7 @@ def test_gpu_gradient_matches_oracle(n):
     P = init_params(4)
     g = grad_step(torch.as_tensor(P, device="cuda"), torch.as_tensor(S, device="cuda"), cfg).cpu().numpy()
     J, go = OM.objective_and_grad(P.astype(np.float64), S, cfg)
-    assert abs(g[-1] / (300 * cfg["ndraw"]) - J) < 1e-4 * max(1.0, J)
+    assert abs(g[-1] / 300 - J) < 1e-4 * max(1.0, abs(J))
     rel = np.linalg.norm(g[:NPARAM] - go) / np.linalg.norm(go)
     assert rel < 2e-3, rel
 
@@ -113,8 +113,8 @@ def test_training_raises_the_objective_and_beats_the_best_fixed_width():
     A.set_temperature(torch.as_tensor(np.repeat(np.linspace(0.5, 3.0, 8).astype(np.float32), B // 8)))
     S = A.init_spins(seed=2)
     lw = train_width(A, S, warmup=60, snapshots=6, per_snapshot=2048, steps=150, batch=8192)
-    J0, J1 = lw.history[0][1], lw.history[-1][1]
-    assert J1 > 1.05 * J0
+    J0, J1 = lw.history[0][1], lw.history[-1][1]          # mean log expected squared jump
+    assert J1 > J0 + 0.05
     wbest, res = best_fixed_width(A, lw.samples)
     assert J1 > max(J for _, J in res)
     tab = lw.table()
