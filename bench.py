@@ -5,7 +5,7 @@ L = 128 periodic, J = 1, h = 0, B = 1024 replicas packed 32 per uint32 word with
 draws, T = 4.5, random +-1 start. A step is one coloured sweep (2 class launches + the counter bump) over all
 replicas; inputs are resident in HBM before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--workload bitwise|ising|heisenberg|xy]
+    python bench.py [--steps K] [--warmup W] [--workload bitwise|ising|heisenberg|xy|learned|nanodisks]
 
 Prints one JSON line. `roofline` is for the dominant kernel (the class-update kernel): algorithmic bytes per
 launch (compulsory read + write of each spin word: 0.25 B per site update) over the launch's average duration,
@@ -37,6 +37,9 @@ WORKLOADS = {
     "xy": "XY triangular L=2048 periodic J1=1, fp32, B=128 replica block (16 temps x 8 in [0.7,1.1])",
     "learned": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, B=8192 (32 temps in [0.5,3.0]): effective "
                "samples/s of the learned cone width over the best fixed width",
+    "nanodisks": "64 Heisenberg sc nanodisks, radius U[8,40], thickness U{4..12} (numpy default_rng(2024)), open, "
+                 "J1=1, bulk DMI D=0.1, Ks=0.3 along the surface normal, 16 replicas each (T in [0.3,1.8]), fp32, "
+                 "every disk in one launch per class, graph-captured sweeps, autotuned schedule",
 }
 
 
@@ -119,7 +122,7 @@ def learned_main(a) -> None:
     s, _, B, T, _ = _make("heisenberg")
     temps = np.unique(T.numpy())
     meas = max(50, a.steps)
-    r = compare_proposals(s, B, temps, warmup=200, therm=max(20, a.warmup), n_meas=meas,
+    r = compare_proposals(s, B, temps, warmup=200, equil=1000, therm=max(20, a.warmup), n_meas=meas,
                           log=lambda m: print(m, flush=True))
     out = {"metric": "effective samples/s, learned vs fixed proposal", "value": r["gain_E"],
            "unit": "ratio of ESS/s of the energy, geometric mean over the temperature ladder", "n_gpus": 1, "steps": meas, "warmup": a.warmup,
@@ -129,14 +132,87 @@ def learned_main(a) -> None:
            "gain_E": r["gain_E"], "gain_M": r["gain_M"], "gain_E_bottleneck": r["gain_E_bottleneck"],
            "gain_M_bottleneck": r["gain_M_bottleneck"], "gain_E_sum": r["gain_E_sum"], "gain_M_sum": r["gain_M_sum"],
            "fixed_width": r["fixed_width"],
+           "vs_uniform_redraw": {k: r["vs_uniform"][k] for k in ("gain_E", "gain_M", "gain_E_bottleneck",
+                                                                   "gain_M_bottleneck")},
            "ess_per_s": {k: {"E": r[k]["ess_E_per_s"], "M": r[k]["ess_M_per_s"],
-                             "updates_per_s": r[k]["updates_per_s"]} for k in ("fixed", "learned")},
+                             "updates_per_s": r[k]["updates_per_s"]} for k in ("uniform", "fixed", "learned")},
            "observables_max_z": r["max_z"], "observables_frac_within_2sigma": r["frac_within_2sigma"],
            "train_seconds": r["train_seconds"]}
     f = REPO / "gpurun_out"
     f.mkdir(exist_ok=True)
     (f / "learned_comparison.json").write_text(json.dumps(r, default=float, indent=1))
     print(json.dumps(out, default=float), flush=True)
+
+
+def nanodisk_main(a) -> None:
+    """Batched uneven systems: all 64 nanodisks x 16 replicas swept together (one JSON line)."""
+    import numpy as np
+    import torch
+
+    import oracle as O
+    from sck_amd.batch import BatchSampler
+    from sck_amd.hamiltonian import nanodisk_set
+    systems = nanodisk_set(64, 8.0, 40.0, 4, 12, seed=2024)
+    reps = 16
+    Bt = BatchSampler(systems, reps, "cuda:0", seed=0, schedule="auto")
+    temps = np.linspace(0.3, 1.8, reps).astype(np.float32)
+    Bt.set_temperature([temps] * len(systems))
+    S = Bt.init_spins(seed=0)
+    S0 = [v.cpu().numpy() for v in Bt.views(S)]
+    Bt.autotune(S, log=lambda m: print(m, file=sys.stderr, flush=True))
+    k = 10
+    rep = Bt.graph(S, k)
+    for _ in range(max(1, a.warmup // k)):
+        rep()
+    torch.cuda.synchronize()
+    nrep = max(1, a.steps // k)
+    steps = nrep * k
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(nrep):
+        rep()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sec = e0.elapsed_time(e1) / 1e3
+    upd = reps * sum(s.n_active for s in systems)
+    value = upd * steps / sec
+    sch = Bt.schedule
+    # dominant kernel: the class launches (2 per sweep) or, for the chain schedule, one launch per k sweeps
+    launches = steps * len(Bt.classes) if sch.kind == "class" else nrep
+    launch_s = sec / launches
+    bytes_per_launch = upd * steps / launches * 24.0
+    achieved = bytes_per_launch / launch_s / 1e9
+    # CPU: the oracle's sequential chain on one replica of the median disk
+    med = int(np.argsort([s.n_active for s in systems])[len(systems) // 2])
+    os_ = O.OracleSystem(systems[med])
+    beta = Bt.beta.cpu().numpy()[int(Bt.rep_off[med]):int(Bt.rep_off[med]) + reps]
+    x = S0[med][[0]].astype(np.float32)
+    t1 = time.perf_counter()
+    sweeps = 0
+    while time.perf_counter() - t1 < 12.0 and sweeps < 2000 and not a.no_cpu_baseline:
+        x, _ = O.chain(os_, x, beta[:1], med, sweeps, 1, reps=[0])
+        sweeps += 1
+    dt = time.perf_counter() - t1
+    out = {"metric": "site updates/s (%HBM roofline)", "value": value, "unit": "site updates/s", "n_gpus": 1,
+           "steps": steps, "warmup": a.warmup, "ms_per_step": sec * 1e3 / steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (random initial spins from the seeded torch CPU generator)",
+           "config": {"workload": WORKLOADS["nanodisks"], "systems": len(systems), "replicas_per_system": reps,
+                      "active_sites": int(sum(s.n_active for s in systems)), "schedule": sch.label(),
+                      "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("nanodisks"),
+                        "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6},
+           "autotune": [[lbl, ms] for lbl, ms in Bt.tuning], "wall_s": wall}
+    if not a.no_cpu_baseline and sweeps:
+        out["cpu_baseline"] = {"value": systems[med].n_active * sweeps / dt, "unit": "site updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"1 replica of the median disk ({systems[med].n_active} sites) x {sweeps} "
+                                         f"sweeps, sequential chain, {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
 
 
 def main() -> None:
@@ -156,6 +232,9 @@ def main() -> None:
     from sck_amd.sampler import Sampler
     if a.workload == "learned":
         learned_main(a)
+        return
+    if a.workload == "nanodisks":
+        nanodisk_main(a)
         return
     s, kw, B, T, alg_bytes = _make(a.workload)
     A = Sampler(s, B, "cuda:0", seed=0, **kw)

@@ -58,6 +58,10 @@ typedef struct sck_args {
     int64_t sweep_offset;     /* sweep index = state[1] + sweep_offset                            */
     int32_t nrep;             /* replicas B (words W for packed/bitwise)                          */
     int32_t nsweep;           /* sweeps one launch of a resident kernel performs (set by the library) */
+    const void *aux;          /* batched systems: the per-system descriptor table                  */
+    const int32_t *tiles;     /* batched systems: the launch's work list (set from sck_launch)     */
+    int32_t ntile;            /* its length                                                        */
+    int32_t pad0;
 } sck_args;
 
 /* One kernel launch of a plan. */
@@ -66,6 +70,9 @@ typedef struct sck_launch {
     uint32_t grid[3];
     uint32_t block[3];
     uint32_t shmem;
+    const int32_t *tiles;     /* NULL, or the work list handed to the kernel as args.tiles / args.ntile */
+    int32_t ntile;
+    int32_t pad0;
 } sck_launch;
 
 /* Launch plans of a system, computed by the host-side generator. */

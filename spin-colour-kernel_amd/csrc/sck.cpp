@@ -163,6 +163,10 @@ namespace {
 int launch(sck_handle *h, const sck_launch &l, const sck_args &a, hipStream_t s) {
     if (l.kernel < 0 || l.kernel >= (int)h->fn.size()) return fail(SCK_E_INVALID, "launch plan names an unknown kernel");
     sck_args copy = a;
+    if (l.tiles) {
+        copy.tiles = l.tiles;
+        copy.ntile = l.ntile;
+    }
     void *params[] = {&copy};
     HIP_TRY(hipModuleLaunchKernel(h->fn[l.kernel], l.grid[0], l.grid[1], l.grid[2], l.block[0], l.block[1],
                                   l.block[2], l.shmem, s, params, nullptr));

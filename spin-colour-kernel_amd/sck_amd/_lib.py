@@ -30,11 +30,13 @@ class Args(C.Structure):
                 ("beta", C.c_void_p), ("table", C.c_void_p), ("state", C.c_void_p), ("width", C.c_void_p),
                 ("site_ku", C.c_void_p), ("site_axis", C.c_void_p), ("obs", C.c_void_p),
                 ("obs_scratch", C.c_void_p), ("sweep_offset", C.c_int64), ("nrep", C.c_int32),
-                ("nsweep", C.c_int32)]
+                ("nsweep", C.c_int32), ("aux", C.c_void_p), ("tiles", C.c_void_p), ("ntile", C.c_int32),
+                ("pad0", C.c_int32)]
 
 
 class LaunchC(C.Structure):
-    _fields_ = [("kernel", C.c_int32), ("grid", C.c_uint32 * 3), ("block", C.c_uint32 * 3), ("shmem", C.c_uint32)]
+    _fields_ = [("kernel", C.c_int32), ("grid", C.c_uint32 * 3), ("block", C.c_uint32 * 3), ("shmem", C.c_uint32),
+                ("tiles", C.c_void_p), ("ntile", C.c_int32), ("pad0", C.c_int32)]
 
 
 class Desc(C.Structure):
@@ -146,6 +148,8 @@ def launches(ls) -> tuple:
             arr[i].grid[k] = int(l.grid[k])
             arr[i].block[k] = int(l.block[k])
         arr[i].shmem = int(l.shmem)
+        arr[i].tiles = getattr(l, "tiles_ptr", None)
+        arr[i].ntile = int(getattr(l, "ntile", 0))
     return arr, len(ls)
 
 

@@ -1,12 +1,14 @@
-"""Effective independent samples per second: learned cone width against the best fixed width, same GPU.
+"""Effective independent samples per second: learned cone width against fixed-width Metropolis, same GPU.
 
-Both samplers start from the same warmed-up spins. The fixed width is the single half-angle with the largest
-training objective on the warm-up records (the learned network's own objective restricted to one constant),
-so the comparison is against a tuned fixed-width Metropolis. tau_int comes from Sokal's window on the energy
-and |M| series of each temperature (sck_amd.observe). Reported gains: per temperature, ESS/s of the learned
-width over the fixed one; their geometric mean over the ladder (the headline); the bottleneck ratio (slowest
+Three proposals run from one common equilibrium state (equilibrated with the learned width, the fastest-mixing
+chain; every proposal keeps detailed balance, so the state stays stationary under each): the uniform redraw
+(the reference's proposal, SCK/generator.py:281-289), the best single fixed half-angle (the largest training
+objective on the warm-up records: the learned network's own objective restricted to one constant, a tuned
+fixed-width Metropolis) and the learned width. tau_int comes from Sokal's window on the energy and |M| series
+of each temperature (sck_amd.observe). Reported gains of one proposal over another: per temperature, the
+ratio of ESS/s; their geometric mean over the ladder (the headline); the bottleneck ratio (slowest
 temperature against slowest temperature); and the ratio of the summed ESS/s, which the fastest (hottest)
-temperatures dominate. Observables E, |M|, Cv and chi carry jackknife errors so the two proposals are checked
+temperatures dominate. Observables E, |M|, Cv and chi carry jackknife errors so the proposals are checked
 against each other within 2 sigma.
 """
 
@@ -37,8 +39,35 @@ def _stats(E, Ma, temps_rep, temps, N):
     return out
 
 
-def compare_proposals(system, B: int, temps, warmup: int = 200, therm: int = 300, n_meas: int = 1000,
-                      every: int = 1, seed: int = 0, train_steps: int = 400, log=print) -> dict:
+def _gains(a: dict, b: dict, ntemp: int) -> dict:
+    """ESS/s of proposal b over proposal a: per temperature (tau_a / tau_b) x (time_a / time_b), their
+    geometric mean over the ladder, the bottleneck ratio and the ratio of the summed ESS/s."""
+    out = {}
+    for key in ("E", "M"):
+        r = np.array([a[f"tau_{key}"][j] / b[f"tau_{key}"][j] * a["seconds"] / b["seconds"] for j in range(ntemp)])
+        out[f"gain_{key}_per_T"] = r.tolist()
+        out[f"gain_{key}"] = float(np.exp(np.log(r).mean()))
+        ta = np.array([a[f"tau_{key}"][j] for j in range(ntemp)])
+        tb = np.array([b[f"tau_{key}"][j] for j in range(ntemp)])
+        out[f"gain_{key}_bottleneck"] = float(ta.max() / tb.max() * a["seconds"] / b["seconds"])
+        out[f"gain_{key}_sum"] = b[f"ess_{key}_per_s"] / a[f"ess_{key}_per_s"]
+    return out
+
+
+def _zscores(a: dict, b: dict) -> np.ndarray:
+    z = []
+    for x, y in zip(a["observables"], b["observables"]):
+        for k in ("E", "M", "Cv", "chi"):
+            err = np.hypot(x[k + "_err"], y[k + "_err"])
+            z.append(abs(x[k] - y[k]) / err if err > 0 else 0.0)
+    return np.asarray(z)
+
+
+def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 1000, therm: int = 50,
+                      n_meas: int = 1000, every: int = 1, seed: int = 0, train_steps: int = 400, log=print) -> dict:
+    """Train the width on `warmup` fixed-width sweeps, equilibrate one state for `equil` sweeps with the
+    learned width, then from that common equilibrium state run `therm` + `n_meas` sweeps of each of: the
+    uniform redraw (the reference's proposal), the best single fixed width, and the learned width."""
     temps = np.asarray(temps, np.float32)
     per = B // len(temps)
     T_rep = np.repeat(temps, per)
@@ -52,12 +81,19 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, therm: int = 300
     lw = train_width(W, S0, warmup=warmup, steps=train_steps, log=log)
     t_train = time.perf_counter() - t0
     w_fixed, scan = best_fixed_width(W, lw.samples)
-    log(f"  fixed-width scan (width, mean squared jump): {[(w, round(j, 5)) for w, j in scan]} -> {w_fixed}")
-    result = dict(system=system.name, B=B, temps=temps.tolist(), warmup=warmup, therm=therm, n_meas=n_meas,
-                  every=every, fixed_width=w_fixed, width_scan=scan, train_seconds=t_train,
+    log(f"  fixed-width scan (width, mean log expected squared jump): "
+        f"{[(w, round(j, 5)) for w, j in scan]} -> {w_fixed}")
+    result = dict(system=system.name, B=B, temps=temps.tolist(), warmup=warmup, equil=equil, therm=therm,
+                  n_meas=n_meas, every=every, fixed_width=w_fixed, width_scan=scan, train_seconds=t_train,
                   train_history=lw.history)
+    # one equilibrium state for all proposals: any chain that keeps detailed balance leaves it stationary
+    Q = Sampler(system, B, "cuda", seed=seed + 7, proposal="learned", learned=lw)
+    Q.set_temperature(torch.as_tensor(T_rep))
     state = S0.clone()
-    for label, kw in (("fixed", dict(proposal="cone", width=w_fixed)), ("learned", dict(proposal="learned", learned=lw))):
+    Q.sweep(state, equil)
+    kinds = (("uniform", dict(proposal="uniform")), ("fixed", dict(proposal="cone", width=w_fixed)),
+             ("learned", dict(proposal="learned", learned=lw)))
+    for label, kw in kinds:
         A = Sampler(system, B, "cuda", seed=seed + 1, **kw)
         A.set_temperature(torch.as_tensor(T_rep))
         S = state.clone()
@@ -71,23 +107,12 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, therm: int = 300
                              observables=_stats(E, Ma, groups, temps, N))
         log(f"  {label:8s}: {secs:7.2f} s, ESS/s (E) {essE / secs:10.1f}, ESS/s (|M|) {essM / secs:10.1f}, "
             f"updates/s {B * N * n_meas * every / secs:.3e}")
-    f, l = result["fixed"], result["learned"]
-    result["gain_E_sum"] = l["ess_E_per_s"] / f["ess_E_per_s"]
-    result["gain_M_sum"] = l["ess_M_per_s"] / f["ess_M_per_s"]
-    # per temperature: ESS/s ratio = (tau_fixed / tau_learned) x (time_fixed / time_learned)
-    for key in ("E", "M"):
-        r = np.array([f[f"tau_{key}"][j] / l[f"tau_{key}"][j] * f["seconds"] / l["seconds"] for j in range(len(temps))])
-        result[f"gain_{key}_per_T"] = r.tolist()
-        result[f"gain_{key}"] = float(np.exp(np.log(r).mean()))              # geometric mean over the ladder
-        tf = np.array([f[f"tau_{key}"][j] for j in range(len(temps))])
-        tl = np.array([l[f"tau_{key}"][j] for j in range(len(temps))])
-        result[f"gain_{key}_bottleneck"] = float(tf.max() / tl.max() * f["seconds"] / l["seconds"])
-    # agreement of the two proposals' observables (z-scores)
-    z = []
-    for a, b in zip(f["observables"], l["observables"]):
-        for k in ("E", "M", "Cv", "chi"):
-            err = np.hypot(a[k + "_err"], b[k + "_err"])
-            z.append(abs(a[k] - b[k]) / err if err > 0 else 0.0)
+    nt = len(temps)
+    result.update(_gains(result["fixed"], result["learned"], nt))
+    result["vs_uniform"] = _gains(result["uniform"], result["learned"], nt)
+    result["fixed_vs_uniform"] = _gains(result["uniform"], result["fixed"], nt)
+    # agreement of the observables: learned against each baseline
+    z = np.concatenate([_zscores(result["fixed"], result["learned"]), _zscores(result["uniform"], result["learned"])])
     result["max_z"] = float(np.max(z))
-    result["frac_within_2sigma"] = float(np.mean(np.asarray(z) < 2.0))
+    result["frac_within_2sigma"] = float(np.mean(z < 2.0))
     return result

@@ -43,7 +43,8 @@ import numpy as np
 from .system import System
 
 __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT8", "MODE_F32", "MODE_PACKED",
-           "MODE_BITWISE", "MODE_BITWISE_INDEP", "BLOCK"]
+           "MODE_BITWISE", "MODE_BITWISE_INDEP", "BLOCK", "BatchModule", "generate_batch", "batch_stride",
+           "BATCH_HDR", "CHAIN_BLOCKS", "COORD_BITS"]
 
 MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP = range(5)
 BLOCK = 256
@@ -58,6 +59,8 @@ class Launch:
     grid: tuple[int, int, int]
     block: tuple[int, int, int] = (BLOCK, 1, 1)
     shmem: int = 0
+    tiles_ptr: int = 0        # device work list handed to the kernel (batched systems), 0 for none
+    ntile: int = 0
 
 
 @dataclass
@@ -201,6 +204,7 @@ typedef struct sck_args {
     void *spins; void *dst; const int8_t *mask; const float *k1; const float *beta; const void *table;
     const int64_t *state; const float *width; const float *site_ku; const float *site_axis;
     double *obs; double *obs_scratch; int64_t sweep_offset; int32_t nrep; int32_t nsweep;
+    const void *aux; const int32_t *tiles; int32_t ntile; int32_t pad0;
 } sck_args;
 
 #define DEV static __device__ __forceinline__
@@ -291,6 +295,7 @@ class _Gen:
         # replica-major storage [b][L3][L2][L1][n][B] (lanes over replicas) or the reference layout
         self.replica_major = bool(replica_major) and mode in (MODE_INT8, MODE_F32)
         self.SS = self.B if self.replica_major else 1
+        self.rt = False               # box dimensions are compile-time constants (runtime: batch kernels)
         self.lines: list[str] = []
 
     # -------------------------------------------------------------------------------------- helpers
@@ -302,22 +307,35 @@ class _Gen:
             esz = 1 if self.mode == MODE_INT8 else 4
             return (f"(*(decltype({ptr}))((const char *)({ptr}) + (int64_t)({site}) * {self.n * self.B * esz} + "
                     f"((uint32_t)rbase + {comp * self.B}u) * {esz}u))")
-        return f"{ptr}[rbase + (int64_t)({comp * self.NS} + {site})]"
+        return f"{ptr}[rbase + (int64_t)({self._mul(comp, self.NS)} + {site})]"
 
     def rbase_expr(self, rep: str) -> str:
         """Offset of replica `rep`'s first spin: the address of component k of site q is
         rbase + (k * NS + q) * SS."""
-        return f"(int64_t){rep}" if self.replica_major else f"(int64_t){rep} * {self.n * self.NS}"
+        return f"(int64_t){rep}" if self.replica_major else f"(int64_t){rep} * {self._mul(self.n, self.NS)}"
 
     def w(self, s: str = "") -> None:
         self.lines.append(s)
+
+    def _mul(self, k: int, X) -> str:
+        """k * X for a compile-time X (folded) or a runtime one (an expression of the batch dimensions)."""
+        if isinstance(X, (int, np.integer)):
+            return str(int(k) * int(X))
+        return "0" if k == 0 else (X if k == 1 else f"{k} * {X}")
+
+    def dims_param(self) -> str:
+        """Extra parameter of the device helpers when the box is a runtime value."""
+        return ", const bdims &D" if self.rt else ""
+
+    def dims_arg(self) -> str:
+        return ", D" if self.rt else ""
 
     def coord(self, var: str, d: int, L: int, periodic: bool):
         """(expression, validity or None) of a neighbour coordinate (SCK/generator.py:48-55)."""
         if d == 0:
             return var, None
         if periodic:
-            if abs(d) < L:
+            if not isinstance(L, (int, np.integer)) or abs(d) < L:
                 return f"wrap_add({var}, {d}, {L})", None
             return f"((({var} + {d}) % {L} + {L}) % {L})", None
         e = f"({var} + {d})"
@@ -331,7 +349,7 @@ class _Gen:
         ey, vy = self.coord("y", d[1], self.L2, self.per[1])
         ez, vz = self.coord("z", d[2], self.L3, self.per[2])
         valid = " && ".join(p for p in (vx, vy, vz) if p) or "true"
-        return f"{bd.alpha2 * self.PL} + (({ez}) * {self.L2} + ({ey})) * {self.L1} + ({ex})", valid
+        return f"{self._mul(bd.alpha2, self.PL)} + (({ez}) * {self.L2} + ({ey})) * {self.L1} + ({ex})", valid
 
     def colour_expr(self, xv: str, a: int) -> str:
         """Colour of site (xv, y, z) of sublattice a (mixed radix of gamma(c, alpha))."""
@@ -372,7 +390,7 @@ class _Gen:
             digits = layouts[a]
             # table lookup: p (Metropolis) or 1 + exp(beta dE) (Barker) of the site's neighbour configuration
             w(f"DEV float isp_{a}(const {T} *__restrict__ src, int64_t rbase, int own, int x, int y, int z, "
-              f"const float *__restrict__ tb, int *s_out) {{")
+              f"const float *__restrict__ tb, int *s_out{self.dims_param()}) {{")
             w(f"    const int s0 = (int){self.at('src', 0, 'own')};")
             w("    *s_out = s0;")
             ng = sum(1 for Jv, _ in groups if Jv != 0.0)
@@ -409,9 +427,10 @@ class _Gen:
             w("}")
             w()
             w(f"DEV int upd_{a}(const sck_args &A, const {T} *__restrict__ src, {T} *__restrict__ dst, int64_t rbase, "
-              f"int rep, int own, int x, int y, int z, uint32_t key) {{")
+              f"int rep, int own, int x, int y, int z, uint32_t key{self.dims_param()}) {{")
             w("    int s0;")
-            w(f"    const float p = isp_{a}(src, rbase, own, x, y, z, (const float *)A.table + (int64_t)rep * {total}, &s0);")
+            w(f"    const float p = isp_{a}(src, rbase, own, x, y, z, (const float *)A.table + (int64_t)rep * {total}, "
+              f"&s0{self.dims_arg()});")
             w("    int acc;")
             w(f"    if ({self.decided_expr('p')}) acc = {self.decided_value('p')};")
             w("    else {")
@@ -423,7 +442,7 @@ class _Gen:
             w("}")
             return
         w(f"DEV int upd_{a}(const sck_args &A, const {T} *__restrict__ src, {T} *__restrict__ dst, int64_t rbase, "
-          f"int rep, int own, int x, int y, int z, uint32_t key) {{")
+          f"int rep, int own, int x, int y, int z, uint32_t key{self.dims_param()}) {{")
         # vector models: fp32 local field in the reference's order
         for k in range(n):
             w(f"    float s{k} = (float){self.at('src', k, 'own')};")
@@ -514,10 +533,12 @@ class _Gen:
     def emit_draw(self, ind: str) -> None:
         w = self.w
         if self.wide:
-            w(f"{ind}const uint64_t ctr = (uint64_t)rep * {self.NS}ull + (uint64_t)own;")
+            ns = f"{self.NS}ull" if not self.rt else f"(uint64_t){self.NS}"
+            w(f"{ind}const uint64_t ctr = (uint64_t)rep * {ns} + (uint64_t)own;")
             w(f"{ind}uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = 0u, c3 = 0u;")
         else:
-            w(f"{ind}uint32_t c0 = (uint32_t)rep * {self.NS}u + (uint32_t)own, c1 = 0u, c2 = 0u, c3 = 0u;")
+            ns = f"{self.NS}u" if not self.rt else f"(uint32_t){self.NS}"
+            w(f"{ind}uint32_t c0 = (uint32_t)rep * {ns} + (uint32_t)own, c1 = 0u, c2 = 0u, c3 = 0u;")
         w(f"{ind}philox10(c0, c1, c2, c3, key, 0u);")
         w(f"{ind}const float u0 = u01(c0);")
         if self.n > 1:
@@ -1304,11 +1325,25 @@ class _Gen:
             w(f"    const int lo = blockIdx.x * {chunk};")
             w(f"    const int hi = min(lo + {chunk}, {self.NS});")
             w("    for (int site = lo + threadIdx.x; site < hi; site += blockDim.x) {")
+        self.emit_site_energy(lambda k, q: f"spin_at(A, rep, {k}, {q})")
+        w("    }")
+        if self.replica_major:
+            w(f"    for (int k = 0; k < {NV}; ++k) A.obs_scratch[((int64_t)rep * gridDim.x + blockIdx.x) * {NV} + k] = acc[k];")
+            w("}")
+        else:
+            self.emit_obs_block_reduce(NV)
+        self.emit_obs_final(NV)
+
+    def emit_site_energy(self, spin) -> None:
+        """Loop body of the observables: site `site`'s coordinates, its spins s<k>, and its share of the energy
+        (bond terms halved) and magnetization added to acc[]; spin(k, q) is the expression of component k of
+        site q of the replica being summed."""
+        w, s, n, b = self.w, self.s, self.n, self.b
         w(f"        const int a = site / {self.PL}, rem = site - a * {self.PL};")
         w(f"        const int x = rem % {self.L1}, row = rem / {self.L1};")
         w(f"        const int z = row / {self.L2}, y = row % {self.L2};")
         for k in range(n):
-            w(f"        const double s{k} = spin_at(A, rep, {k}, site);")
+            w(f"        const double s{k} = {spin(k, 'site')};")
         w("        double e = 0.0;")
         for a in range(b):
             w(f"        if (a == {a}) {{")
@@ -1323,7 +1358,7 @@ class _Gen:
                 q, ok = self.nb_index(i)
                 w(f"            if ({ok}) {{ const int q = {q};")
                 for l in range(n):
-                    w(f"                const double n{l} = spin_at(A, rep, {l}, q);")
+                    w(f"                const double n{l} = {spin(l, 'q')};")
                 for k in range(n):
                     terms = [f"{_f64(M[k, l])} * n{l}" for l in range(n) if M[k, l] != 0.0]
                     if terms:
@@ -1345,13 +1380,6 @@ class _Gen:
             sn = " + ".join(f"s{k} * (double)A.site_axis[(int64_t)site * {n} + {k}]" for k in range(n))
             w(f"        {{ const double sn = {sn}; e -= (double)A.site_ku[site] * sn * sn; }}")
         w("        acc[0] += e;")
-        w("    }")
-        if self.replica_major:
-            w(f"    for (int k = 0; k < {NV}; ++k) A.obs_scratch[((int64_t)rep * gridDim.x + blockIdx.x) * {NV} + k] = acc[k];")
-            w("}")
-        else:
-            self.emit_obs_block_reduce(NV)
-        self.emit_obs_final(NV)
 
     def emit_obs_block_reduce(self, NV: int) -> None:
         w = self.w
@@ -1457,6 +1485,215 @@ class _Gen:
             mod.resident = Launch(ki["sck_resident"], (self.B, 1, 1), (self.RESIDENT_THREADS, 1, 1))
         mod.key = hashlib.sha256(src.encode()).hexdigest()[:16]
         return mod
+
+
+# ------------------------------------------------------------------------------------ batched systems
+BATCH_HDR = 4                 # int64 header of the batch table: systems, site-list offset (uint32 words), classes, 0
+CLASS_BOUND = 256             # launch bound of the batch class kernels (schedules use 64 .. 256 threads)
+CHAIN_BLOCKS = (256, 1024)    # workgroup sizes of the chain kernels
+COORD_BITS = 10               # packed site coordinates x | y << 10 | z << 20
+
+
+def batch_stride(nclass: int) -> int:
+    """int64 words of one system's descriptor: L1, L2, L3, replicas, first global replica, Philox seed, spin
+    offset (elements), site offset, then (first, count) of each class's site list."""
+    return 8 + 2 * nclass
+
+
+@dataclass
+class BatchModule:
+    src: str
+    kernels: list[str]
+    classes: list[tuple[int, int]]
+    table_entries: int = 0
+    key: str = ""
+
+
+class _BatchGen(_Gen):
+    """Kernels for a batch of systems that share one Hamiltonian structure (model, crystal, couplings, extra
+    matrices, field, anisotropy form, colouring classes) but differ in box, mask and per-site constants. The
+    box is a runtime value read from the system's descriptor; each system keeps its own Philox seed, its own
+    counter layout rep * nsite + own and its own spin array in the reference layout, so a system in the batch
+    runs exactly the chain it runs alone."""
+
+    def __init__(self, systems, mode, proposal, learned_meta, wide):
+        super().__init__(systems[0], mode, 1, proposal, learned_meta, replica_major=False)
+        self.rt = True
+        self.L1, self.L2, self.L3, self.PL, self.NS = "D.L1", "D.L2", "D.L3", "D.PL", "D.NS"
+        self.wide = bool(wide)
+        self.T = "int8_t" if mode == MODE_INT8 else "float"
+        self.stride = batch_stride(len(self.classes))
+
+    def emit_batch_helpers(self) -> None:
+        w, n, T = self.w, self.n, self.T
+        _, _, total = ising_layout(self.s) if n == 1 else ([], [], 0)
+        w("typedef struct bdims { int L1, L2, L3, PL, NS; } bdims;")
+        w()
+        w("DEV const int64_t *bdesc(const sck_args &G, int s) {")
+        w(f"    return (const int64_t *)G.aux + {BATCH_HDR} + (int64_t)s * {self.stride};")
+        w("}")
+        w()
+        w("DEV const uint32_t *blist(const sck_args &G) {")
+        w("    return (const uint32_t *)G.aux + ((const int64_t *)G.aux)[1];")
+        w("}")
+        w()
+        w("// the system's box and its arguments: pointers moved to its spins, site constants and first replica")
+        w("DEV void bsys(const sck_args &G, const int64_t *d, bdims &D, sck_args &A) {")
+        w("    D.L1 = (int)d[0]; D.L2 = (int)d[1]; D.L3 = (int)d[2];")
+        w("    D.PL = D.L1 * D.L2 * D.L3;")
+        w(f"    D.NS = {self.b} * D.PL;")
+        w("    A = G;")
+        w(f"    A.spins = ({T} *)G.spins + d[6];")
+        w("    A.mask = G.mask + d[7];")
+        w("    A.k1 = G.k1 + d[7];")
+        w("    A.site_ku = G.site_ku + d[7];")
+        w(f"    A.site_axis = G.site_axis + d[7] * {n};")
+        w("    A.beta = G.beta + d[4];")
+        if self.proposal == "cone":
+            w("    A.width = G.width + d[4];")
+        if n == 1:
+            w(f"    A.table = (const float *)G.table + d[4] * {total};")
+        w("}")
+        w()
+        w("DEV uint32_t bkey(int64_t seed, int64_t sweep) {")
+        w("    return (uint32_t)((seed * 1000003LL + sweep * 7919LL + 12345LL) % 2147483629LL);")
+        w("}")
+        w()
+        w("DEV void bcoord(uint32_t c, int &x, int &y, int &z) {")
+        w(f"    x = (int)(c & {(1 << COORD_BITS) - 1}u); y = (int)((c >> {COORD_BITS}) & {(1 << COORD_BITS) - 1}u); "
+          f"z = (int)(c >> {2 * COORD_BITS});")
+        w("}")
+        w()
+
+    def emit_class_pass(self, k: int, a: int, lo: str, hi: str, step: str, ind: str) -> None:
+        """Update the sites list[lo .. hi) of class k (sublattice a), one per lane, `step` apart."""
+        w = self.w
+        w(f"{ind}for (int i = {lo}; i < {hi}; i += {step}) {{")
+        w(f"{ind}    int x, y, z; bcoord(list[i], x, y, z);")
+        w(f"{ind}    const int own = {self._mul(a, 'D.PL')} + (z * D.L2 + y) * D.L1 + x;")
+        w(f"{ind}    upd_{a}(A, S, S, rbase, rep, own, x, y, z, key, D);")
+        w(f"{ind}}}")
+
+    def emit_chain_prologue(self, idx: str) -> None:
+        w, n, T = self.w, self.n, self.T
+        w(f"    const int32_t *tl = G.tiles + 4 * (int64_t)({idx});")
+        w("    const int sys = tl[0], rep = tl[1];")
+        w("    const int64_t *d = bdesc(G, sys);")
+        w("    bdims D; sck_args A; bsys(G, d, D, A);")
+        w(f"    {T} *S = ({T} *)A.spins;")
+        w(f"    const int64_t rbase = (int64_t)rep * {n} * D.NS;")
+        w("    const uint32_t *list = blist(G);")
+
+    def generate_batch(self) -> BatchModule:
+        s, w = self.s, self.w
+        w(HEADER)
+        w("#pragma clang fp contract(off)")
+        w(f"// batch of systems sharing one structure: n={s.n} crystal {s.crystal.name} b={s.b} "
+          f"periodic={s.periodic}; classes {self.classes}; mode {self.mode}; proposal {self.proposal}")
+        w()
+        self.emit_batch_helpers()
+        for a in range(self.b):
+            self.emit_update_plain(a)
+            w()
+        # class kernels: persistent workgroups stride over the tiles (system, replica, list range) of one class
+        for k, (c, a) in enumerate(self.classes):
+            w(f'extern "C" __global__ void __launch_bounds__({CLASS_BOUND}) sck_bclass_{k}(sck_args G) {{')
+            w("    const int64_t sweep = G.state[1] + G.sweep_offset;")
+            w("    for (int t = blockIdx.x; t < G.ntile; t += gridDim.x) {")
+            w("        const int32_t *tl = G.tiles + 4 * (int64_t)t;")
+            w("        const int sys = tl[0], rep = tl[1], lo = tl[2], hi = tl[3];")
+            w("        const int64_t *d = bdesc(G, sys);")
+            w("        bdims D; sck_args A; bsys(G, d, D, A);")
+            w("        const uint32_t key = bkey(d[5], sweep);")
+            w(f"        {self.T} *S = ({self.T} *)A.spins;")
+            w(f"        const int64_t rbase = (int64_t)rep * {self.n} * D.NS;")
+            w("        const uint32_t *list = blist(G);")
+            self.emit_class_pass(k, a, "lo + (int)threadIdx.x", "hi", "(int)blockDim.x", "        ")
+            w("    }")
+            w("}")
+            w()
+        # chain kernels: one workgroup runs all classes of A.nsweep sweeps of one (system, replica) chain;
+        # chains are independent, so barriers inside the workgroup replace the kernel boundaries
+        for nt in CHAIN_BLOCKS:
+            w(f'extern "C" __global__ void __launch_bounds__({nt}) sck_bchain_{nt}(sck_args G) {{')
+            w("    if ((int)blockIdx.x >= G.ntile) return;")
+            self.emit_chain_prologue("blockIdx.x")
+            w("    if (tl[3] < 0) return;                    // an empty slot of the work list")
+            w("    for (int sw = 0; sw < G.nsweep; ++sw) {")
+            w("        const uint32_t key = bkey(d[5], G.state[1] + G.sweep_offset + sw);")
+            for k, (c, a) in enumerate(self.classes):
+                w("        {")
+                w(f"            const int lo = (int)d[{8 + 2 * k}], hi = lo + (int)d[{9 + 2 * k}];")
+                self.emit_class_pass(k, a, "lo + (int)threadIdx.x", "hi", "(int)blockDim.x", "            ")
+                w("        }")
+                w("        __syncthreads();")
+            w("    }")
+            w("}")
+            w()
+        # sequential chain: one thread per (system, replica), classes in order, sites in raster order
+        w('extern "C" __global__ void __launch_bounds__(64) sck_bseq(sck_args G) {')
+        w("    const int g = blockIdx.x * blockDim.x + threadIdx.x;")
+        w("    if (g >= G.ntile) return;")
+        self.emit_chain_prologue("g")
+        w("    const uint32_t key = bkey(d[5], G.state[1] + G.sweep_offset);")
+        for k, (c, a) in enumerate(self.classes):
+            w("    {")
+            w(f"        const int lo = (int)d[{8 + 2 * k}], hi = lo + (int)d[{9 + 2 * k}];")
+            self.emit_class_pass(k, a, "lo", "hi", "1", "        ")
+            w("    }")
+        w("}")
+        w()
+        self.emit_batch_observe()
+        w('extern "C" __global__ void sck_bump(sck_args A) {')
+        w("    if (threadIdx.x == 0 && blockIdx.x == 0) ((int64_t *)A.state)[1] += A.sweep_offset;")
+        w("}")
+        src = "\n".join(self.lines) + "\n"
+        kernels = [f"sck_bclass_{k}" for k in range(len(self.classes))] + \
+                  [f"sck_bchain_{nt}" for nt in CHAIN_BLOCKS] + ["sck_bseq", "sck_bobs", "sck_bump"]
+        _, _, total = ising_layout(s) if s.n == 1 else ([], [], 0)
+        return BatchModule(src, kernels, self.classes, table_entries=total,
+                           key=hashlib.sha256(src.encode()).hexdigest()[:16])
+
+    def emit_batch_observe(self) -> None:
+        """sck_bobs: float64 energy and magnetization of each (system, replica), one workgroup each (tiles in
+        global replica order), written to obs[g] and obs[R + g * b n ...] like the single-system kernels."""
+        w, n, b = self.w, self.n, self.b
+        NV = 1 + b * n
+        w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_bobs(sck_args G) {{')
+        w("    const int g = blockIdx.x;")
+        w("    if (g >= G.ntile) return;")
+        self.emit_chain_prologue("g")
+        w(f"    double acc[{NV}];")
+        w(f"    for (int k = 0; k < {NV}; ++k) acc[k] = 0.0;")
+        w("    for (int site = threadIdx.x; site < D.NS; site += blockDim.x) {")
+        self.emit_site_energy(lambda k, q: f"(double)S[rbase + (int64_t){self._mul(k, 'D.NS')} + {q}]")
+        w("    }")
+        w(f"    __shared__ double red[{BLOCK // 64}][{NV}];")
+        w(f"    for (int k = 0; k < {NV}; ++k) {{")
+        w("        double v = acc[k];")
+        w("        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);")
+        w("        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;")
+        w("    }")
+        w("    __syncthreads();")
+        w(f"    if (threadIdx.x < {NV}) {{")
+        w("        double v = 0.0;")
+        w(f"        for (int i = 0; i < {BLOCK // 64}; ++i) v += red[i][threadIdx.x];")
+        w("        if (threadIdx.x == 0) G.obs[g] = v;")
+        w(f"        else G.obs[G.nrep + (int64_t)g * {NV - 1} + (threadIdx.x - 1)] = v;")
+        w("    }")
+        w("}")
+        w()
+
+
+def generate_batch(systems, mode: int, proposal: str = "uniform", learned_meta=None, wide: bool = False) -> BatchModule:
+    """HIP source of the batch kernels for systems sharing one Hamiltonian structure (checked by the caller)."""
+    if mode not in (MODE_INT8, MODE_F32):
+        raise ValueError("batched systems run unpacked int8 (Ising) or float32 spins")
+    if proposal not in ("uniform", "cone", "learned"):
+        raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
+    if proposal != "uniform" and systems[0].n == 1:
+        raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
+    return _BatchGen(list(systems), mode, proposal, learned_meta, wide).generate_batch()
 
 
 def fits_resident(sys_: System, mode: int) -> bool:

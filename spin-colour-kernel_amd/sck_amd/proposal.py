@@ -192,7 +192,7 @@ def best_fixed_width(sampler, samples: torch.Tensor, widths=(0.25, 0.5, 0.75, 1.
     """The single cone half-angle with the largest training objective (mean log expected squared jump) over
     the training records: the fixed-width baseline the learned width is judged against (the same objective
     restricted to one constant)."""
-    best, bestJ = None, -1.0
+    best, bestJ = None, -math.inf
     cfg = mlp_config(sampler, ndraw=ndraw)
     dev = samples.device
     res = []
