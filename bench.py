@@ -75,35 +75,54 @@ def prebuilt_specs():
     return [(s, G.MODE_BITWISE_INDEP, B)]
 
 
-def cpu_baseline(workload: str, s, A, S0, budget_s: float = 12.0) -> dict:
-    """Time the CPU oracle (sequential chain in C, one thread) on a bounded sample of the same workload."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(workload: str, s, A, S0, budget_s: float = 10.0) -> dict:
+    """Time the CPU oracle (the sequential chain in C) on a bounded sample of the same workload: one thread
+    (the reported value) and, as a second row, one thread per replica (word) on all usable host cores."""
     import numpy as np
 
     import oracle as O
     os_ = O.OracleSystem(s)
-    if A.bitwise:
-        x = O.unpack(S0[0:1], 32, s.site_mask())[:, 0]
-        tab = A.ptab.cpu().numpy()[0:1]
+    ncores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+
+    def run(nrep: int, threads: int, budget: float):
+        if A.bitwise:
+            x = O.unpack(S0[0:nrep], 32 * nrep, s.site_mask())[:, 0]
+            tab = A.ptab.cpu().numpy()[0:nrep]
+        else:
+            beta = A.beta.cpu().numpy()
+            x = S0[list(range(nrep))].astype(np.float32)
         t0 = time.perf_counter()
         sweeps = 0
-        while time.perf_counter() - t0 < budget_s and sweeps < 50:
-            x = O.bitwise_chain(os_, x, tab, 0, sweeps, 1, independent=A.independent, threads=1)
+        while time.perf_counter() - t0 < budget and sweeps < 1000:
+            if A.bitwise:
+                x = O.bitwise_chain(os_, x, tab, 0, sweeps, 1, independent=A.independent, threads=threads)
+            else:
+                x, _ = O.chain(os_, x, beta, 0, sweeps, 1, reps=list(range(nrep)), threads=threads)
             sweeps += 1
         dt = time.perf_counter() - t0
-        return {"value": 32 * s.n_active * sweeps / dt, "unit": "site updates/s", "cores": 1, "kind": "port",
-                "sample": f"1 word (32 replicas) x {s.n_active} sites x {sweeps} sweeps, sequential chain, "
-                          f"{dt:.1f} s"}
-    reps = [0]
-    beta = A.beta.cpu().numpy()
-    x = S0[reps].astype(np.float32)
-    t0 = time.perf_counter()
-    sweeps = 0
-    while time.perf_counter() - t0 < budget_s and sweeps < 1000:
-        x, _ = O.chain(os_, x, beta, 0, sweeps, 1, reps=reps)
-        sweeps += 1
-    dt = time.perf_counter() - t0
-    return {"value": len(reps) * s.n_active * sweeps / dt, "unit": "site updates/s", "cores": 1, "kind": "port",
-            "sample": f"{len(reps)} replica x {s.n_active} sites x {sweeps} sweeps, sequential chain, {dt:.1f} s"}
+        per = 32 if A.bitwise else 1
+        return nrep * per * s.n_active * sweeps / dt, sweeps, dt
+
+    unit = "word (32 replicas)" if A.bitwise else "replica"
+    ncores = min(ncores, int(S0.shape[0]))
+    v1, sw1, dt1 = run(1, 1, budget_s)
+    vn, swn, dtn = run(ncores, ncores, budget_s)
+    return {"value": v1, "unit": "site updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 {unit} x {s.n_active} sites x {sw1} sweeps, sequential chain, {dt1:.1f} s",
+            "cpu_model": _cpu_model(),
+            "all_cores": {"value": vn, "cores": ncores,
+                          "sample": f"{ncores} x 1 {unit} (one per thread, OpenMP) x {s.n_active} sites x {swn} "
+                                    f"sweeps, {dtn:.1f} s"}}
 
 
 def traffic_from_profiles(workload: str):
@@ -183,7 +202,7 @@ def nanodisk_main(a) -> None:
     # dominant kernel: the class launches (2 per sweep) or, for the chain schedule, one launch per k sweeps
     launches = steps * len(Bt.classes) if sch.kind == "class" else nrep
     launch_s = sec / launches
-    bytes_per_launch = upd * steps / launches * 24.0
+    bytes_per_launch = upd * steps / launches * 29.0        # 24 B spin read + write, 1 B mask, 4 B site K
     achieved = bytes_per_launch / launch_s / 1e9
     # CPU: the oracle's sequential chain on one replica of the median disk
     med = int(np.argsort([s.n_active for s in systems])[len(systems) // 2])

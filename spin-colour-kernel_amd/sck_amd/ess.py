@@ -33,7 +33,8 @@ def _stats(E, Ma, temps_rep, temps, N):
         e, m = E[:, sel], Ma[:, sel]
         Em, Ee = jackknife(lambda x: x.mean() / N, e)
         Mm, Me = jackknife(lambda x: x.mean(), m)
-        (cv, cve), (chi, chie) = cv_chi(e, m, float(T), N)
+        # pooled variances: the per-chain estimator's bias grows with tau, which differs between proposals
+        (cv, cve), (chi, chie) = cv_chi(e, m, float(T), N, pooled=True)
         out.append(dict(T=float(T), E=float(Em), E_err=float(Ee), M=float(Mm), M_err=float(Me), Cv=cv, Cv_err=float(cve),
                         chi=chi, chi_err=float(chie)))
     return out

@@ -84,14 +84,21 @@ def jackknife(stat, data: np.ndarray, blocks: int = 16):
     return full, err
 
 
-def cv_chi(E: np.ndarray, Mabs: np.ndarray, T: float, N: float):
-    """Per-replica Cv = var(E)/(T^2 N) and chi = N var(|M|)/T from series (time, chains), averaged over chains
-    with their jackknife errors."""
+def cv_chi(E: np.ndarray, Mabs: np.ndarray, T: float, N: float, pooled: bool = False):
+    """Cv = var(E)/(T^2 N) and chi = N var(|M|)/T from series (time, chains) with jackknife errors over chains.
+
+    pooled=False: the variance of each chain's own series, averaged over chains (the reference's estimator,
+    SCK/simulate.py:521-526); it is biased low by about 2 tau_int / n for a series of n correlated samples.
+    pooled=True: one variance over every (time, chain) sample of chains that sample the same equilibrium,
+    free of that tau-dependent bias (used to compare proposals with different tau)."""
+    def var(x):
+        return x.reshape(-1).var() if pooled else x.var(axis=0).mean()
+
     def cv(e):
-        return float((e.var(axis=0) / (T * T * N)).mean())
+        return float(var(e) / (T * T * N))
 
     def chi(m):
-        return float((N * m.var(axis=0) / T).mean())
+        return float(N * var(m) / T)
     return jackknife(cv, E), jackknife(chi, Mabs)
 
 

@@ -4,10 +4,12 @@ CPU: the surface normals against a brute-force loop over sites, the DMI matrix a
 D r . (S_i x S_j), the batch host tables (compacted class lists cover every active site of every class once;
 every schedule's work list covers every (system, replica, class) range once; the XCD interleave keeps a
 chain's tiles on one XCD), and the structure check.
+The oracle's surface-anisotropy terms are pinned by mathematics: the energy of uniform states in closed form
+and the single-spin distribution by quadrature over the sphere.
 GPU: a batch equals independent Samplers of its systems bit for bit under every schedule the autotuner can
 pick, for Heisenberg (uniform and cone proposals) and Ising; the batch's parallel sweep equals its own
 sequential chain; batch observables equal the single-system kernels; graph replays equal eager sweeps; a
-nanodisk's decisions match the oracle's.
+nanodisk's decisions match the oracle's; the sweep samples the single-spin distribution by quadrature.
 """
 
 import itertools
@@ -85,6 +87,68 @@ def test_nanodisk_shapes():
                                                               if x * x + y * y <= 64))
     big = nanodisk(40, 12)
     assert big.box == (81, 81, 12)
+
+
+def test_oracle_energy_of_uniform_nanodisk_states_is_the_closed_form():
+    """All spins along u: DMI vanishes (S x S = 0), exchange gives -J1 per active bond, and the surface term
+    -Ks (n . u)^2 per surface site."""
+    s = nanodisk(5.2, 3)
+    m = s.site_mask()
+    bonds = int((m[0, :, :, :-1] & m[0, :, :, 1:]).sum()) + int((m[0, :, :-1, :] & m[0, :, 1:, :]).sum()) \
+        + int((m[0, :-1] & m[0, 1:]).sum())
+    ku = s.site_ku.reshape(-1)
+    ax = s.site_axis.reshape(-1, 3).astype(np.float64)
+    for u in (np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]), np.array([0.6, 0.0, 0.8])):
+        x = np.zeros((1, 3) + s.shape, np.float32)
+        for k in range(3):
+            x[0, k] = u[k] * m
+        E, M = O.observe(O.OracleSystem(s), x)
+        uu = x[0].reshape(3, -1).T.astype(np.float64)
+        want = -1.0 * bonds - float((ku * ((ax * uu).sum(1)) ** 2).sum())
+        assert abs(E[0] - want) < 1e-4 * abs(want), (u, E[0], want)
+        assert np.allclose(M[0, 0], u * m.sum(), rtol=1e-6)
+
+
+def _site_anisotropy_spin():
+    ax = np.array([0.6, 0.0, 0.8])
+    s = sy.System("single spin, site anisotropy", 3, cr.make("sc"), (1, 1, 1), (False,) * 3, {1: 1.0},
+                  h=(0.0, 0.0, 0.4), site_ku=np.full((1, 1, 1, 1), 0.9, np.float32),
+                  site_axis=np.broadcast_to(ax.astype(np.float32), (1, 1, 1, 1, 3)).copy())
+    return s, ax, 0.9, 0.4, 0.5
+
+
+def _sphere_average(ax, Ks, h, T):
+    """(<s_z>, <(s.a)^2>) under exp(beta (h s_z + Ks (s.a)^2)) by quadrature over the sphere."""
+    th = np.linspace(0, np.pi, 1201)
+    ph = np.linspace(0, 2 * np.pi, 1201)
+    TH, PH = np.meshgrid(th, ph, indexing="ij")
+    x, y, z = np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)
+    dot = ax[0] * x + ax[1] * y + ax[2] * z
+    wgt = np.exp((h * z + Ks * dot * dot) / T) * np.sin(TH)
+    Z = np.trapezoid(np.trapezoid(wgt, ph, axis=1), th)
+    return (np.trapezoid(np.trapezoid(wgt * z, ph, axis=1), th) / Z,
+            np.trapezoid(np.trapezoid(wgt * dot * dot, ph, axis=1), th) / Z)
+
+
+def test_oracle_site_anisotropy_single_spin_matches_quadrature():
+    """Pins the oracle's per-site anisotropy term (sign, axis) by the exact single-spin distribution."""
+    s, ax, Ks, h, T = _site_anisotropy_spin()
+    B = 2048
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(B, 3, 1, 1, 1, 1)).astype(np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    os_ = O.OracleSystem(s)
+    beta = np.full(B, 1.0 / T, np.float32)
+    x, _ = O.chain(os_, x, beta, 4, 0, 200, threads=8)
+    sz, sa = [], []
+    for i in range(20):
+        x, _ = O.chain(os_, x, beta, 4, 200 + 5 * i, 5, threads=8)
+        v = x[:, :, 0, 0, 0, 0].astype(np.float64)
+        sz.append(v[:, 2].mean())
+        sa.append(((v @ ax) ** 2).mean())
+    ez, ea = _sphere_average(ax, Ks, h, T)
+    for series, exact in ((sz, ez), (sa, ea)):
+        assert abs(np.mean(series) - exact) < 3 * np.std(series) / math.sqrt(len(series)) + 2e-3, exact
 
 
 def _layout(systems, reps, seeds=None):
@@ -314,3 +378,26 @@ def test_nanodisk_decisions_match_oracle():
         near = np.abs(mg.reshape(bad.shape)) < 1e-5
         assert not np.any(bad & ~near), f"class {c}: {int((bad & ~near).sum())} decisions differ"
         S.copy_(torch.as_tensor(ref, device=DEV))
+
+
+@pytest.mark.gpu
+def test_single_spin_site_anisotropy_distribution_is_exact():
+    """One Heisenberg spin with a site anisotropy Ks along a tilted axis a and a field h along z:
+    <s_z> and <(s.a)^2> under e^{beta (h s_z + Ks (s.a)^2)} by quadrature over the sphere."""
+    from sck_amd.sampler import Sampler
+    s, ax, Ks, h, T = _site_anisotropy_spin()
+    B = 4096
+    A = Sampler(s, B, DEV, seed=4)
+    A.set_temperature(T)
+    S = A.init_spins(seed=1)
+    A.sweep(S, 200)
+    sz, sa = [], []
+    axt = torch.as_tensor(ax, device=DEV)
+    for _ in range(40):
+        A.sweep(S, 5)
+        v = S[:, :, 0, 0, 0, 0].double()
+        sz.append(v[:, 2].mean().item())
+        sa.append(((v @ axt) ** 2).mean().item())
+    ez, ea = _sphere_average(ax, Ks, h, T)
+    for series, exact in ((sz, ez), (sa, ea)):
+        assert abs(np.mean(series) - exact) < 3 * np.std(series) / math.sqrt(len(series)) + 2e-3, exact

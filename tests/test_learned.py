@@ -76,6 +76,23 @@ def test_integrated_tau_known_answers():
     assert 0.8 < w < 1.2
 
 
+def test_pooled_variance_is_free_of_the_autocorrelation_bias():
+    """AR(1) chains x_t = phi x_{t-1} + eps of known variance 1 / (1 - phi^2): the per-chain estimator (the
+    reference's) is low by about 2 tau / n, the pooled one is not."""
+    from sck_amd.observe import cv_chi
+    rng = np.random.default_rng(0)
+    phi, n, c = 0.9, 400, 256
+    x = np.zeros((n, c))
+    x[0] = rng.normal(size=c) / math.sqrt(1 - phi * phi)
+    for t in range(1, n):
+        x[t] = phi * x[t - 1] + rng.normal(size=c)
+    true = 1 / (1 - phi * phi)
+    (per, per_e), _ = cv_chi(x, x, 1.0, 1.0)
+    (pool, pool_e), (chi, _) = cv_chi(x, x, 1.0, 1.0, pooled=True)
+    assert abs(pool - true) < 3 * pool_e and chi == pool
+    assert per < true - 2 * per_e
+
+
 def test_jackknife_error_of_the_mean():
     rng = np.random.default_rng(1)
     x = rng.normal(size=4000)
