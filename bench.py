@@ -34,7 +34,7 @@ WORKLOADS = {
     "bitwise": "bitwise Ising sc L=128 periodic, B=1024 (32 words), independent draws, T=4.5",
     "ising": "Ising square L=64 periodic, int8, B=512 (8 temps x 64 in [1.5,3.5]), seed 0",
     "heisenberg": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, fp32, B=8192 (32 temps x 256 in [0.5,3.0])",
-    "xy": "XY triangular L=2048 periodic J1=1, fp32, B=128 replica block (16 temps x 8 in [0.7,1.1])",
+    "xy": "XY triangular L=2048 periodic J1=1, fp32, B=2048 (16 temps x 128 in [0.7,1.1]), replica-major",
     "learned": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, B=8192 (32 temps in [0.5,3.0]): effective "
                "samples/s of the learned cone width over the best fixed width",
     "nanodisks": "64 Heisenberg sc nanodisks, radius U[8,40], thickness U{4..12} (numpy default_rng(2024)), open, "
@@ -63,16 +63,26 @@ def _make(workload: str):
         return s, {}, 8192, T, 24.0
     if workload == "xy":
         s = sy.xy("triangular", 2048)
-        T = torch.as_tensor(np.repeat(np.linspace(0.7, 1.1, 16).astype(np.float32), 8))
-        return s, {}, 128, T, 16.0
+        T = torch.as_tensor(np.repeat(np.linspace(0.7, 1.1, 16).astype(np.float32), 128))
+        return s, {}, 2048, T, 16.0
     raise SystemExit(f"unknown workload {workload!r}; known: {', '.join(WORKLOADS)}")
 
 
 def prebuilt_specs():
-    """(system, mode, batch) of the default workload, for the build step's kernel pre-compilation."""
+    """(system, mode, batch, replica_major) of every bench workload, for the build step's kernel
+    pre-compilation (the layout is the one the Sampler picks for it)."""
     from sck_amd import hipgen as G
-    s, kw, B, _, _ = _make("bitwise")
-    return [(s, G.MODE_BITWISE_INDEP, B)]
+    out = []
+    for wl in ("bitwise", "ising", "heisenberg", "xy"):
+        s, kw, B, _, _ = _make(wl)
+        if kw.get("bitwise"):
+            out.append((s, G.MODE_BITWISE_INDEP, B, False))
+        elif s.n == 1:
+            out.append((s, G.MODE_INT8, B, False))
+        else:
+            rm = B % 64 == 0 and not G.fits_resident(s, G.MODE_F32)
+            out.append((s, G.MODE_F32, B, rm))
+    return out
 
 
 def _cpu_model() -> str:
@@ -125,6 +135,31 @@ def cpu_baseline(workload: str, s, A, S0, budget_s: float = 10.0) -> dict:
                                     f"sweeps, {dtn:.1f} s"}}
 
 
+def device_xy_spins(A, seed: int = 0):
+    """Random XY spins (uniform angles) drawn on the device one row block at a time with torch's seeded CUDA
+    generator, straight into the sampler's storage (a full-size batch is too large to draw on the host)."""
+    import math
+
+    import torch
+    s = A.sys
+    g = torch.Generator(device=A.device).manual_seed(int(seed))
+    if A.replica_major:
+        store = torch.empty((s.b, s.L3, s.L2, s.L1, 2, A.B), dtype=torch.float32, device=A.device)
+        for z in range(s.L3):
+            for y0 in range(0, s.L2, 64):
+                y1 = min(s.L2, y0 + 64)
+                phi = torch.rand((s.b, y1 - y0, s.L1, A.B), generator=g, device=A.device) * (2 * math.pi)
+                store[:, z, y0:y1, :, 0] = torch.cos(phi)
+                store[:, z, y0:y1, :, 1] = torch.sin(phi)
+        return store.permute(5, 4, 0, 1, 2, 3)
+    S = torch.empty((A.B, 2) + s.shape, dtype=torch.float32, device=A.device)
+    for r0 in range(0, A.B, 16):
+        phi = torch.rand((min(16, A.B - r0),) + s.shape, generator=g, device=A.device) * (2 * math.pi)
+        S[r0:r0 + 16, 0] = torch.cos(phi)
+        S[r0:r0 + 16, 1] = torch.sin(phi)
+    return S
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per launch of the class kernel from the committed rocprofv3 counter summary, if any."""
     f = REPO / "profiles" / f"traffic_{workload}.json"
@@ -173,12 +208,18 @@ def nanodisk_main(a) -> None:
     from sck_amd.hamiltonian import nanodisk_set
     systems = nanodisk_set(64, 8.0, 40.0, 4, 12, seed=2024)
     reps = 16
-    Bt = BatchSampler(systems, reps, "cuda:0", seed=0, schedule="auto")
+    sch = "auto"
+    if a.schedule != "auto":
+        from sck_amd.batch import Schedule
+        kind, block, tile, order, grid = a.schedule.split("/")
+        sch = Schedule(kind, int(block), int(tile), order, int(grid))
+    Bt = BatchSampler(systems, reps, "cuda:0", seed=0, schedule=sch)
     temps = np.linspace(0.3, 1.8, reps).astype(np.float32)
     Bt.set_temperature([temps] * len(systems))
     S = Bt.init_spins(seed=0)
     S0 = [v.cpu().numpy() for v in Bt.views(S)]
-    Bt.autotune(S, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if sch == "auto":
+        Bt.autotune(S, log=lambda m: print(m, file=sys.stderr, flush=True))
     k = 10
     rep = Bt.graph(S, k)
     for _ in range(max(1, a.warmup // k)):
@@ -241,6 +282,10 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="bitwise", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (the reference for the HBM counters)")
+    ap.add_argument("--schedule", default="auto",
+                    help="nanodisks: 'auto' (autotune) or kind/block/tile/order/grid, e.g. class/256/256/xcd/0")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: the sweep path has no multi-GPU component "
@@ -249,6 +294,14 @@ def main() -> None:
     import torch
 
     from sck_amd.sampler import Sampler
+    if a.calibrate_copy:
+        x = torch.ones(2 ** 28, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "copies": 3}))
+        return
     if a.workload == "learned":
         learned_main(a)
         return
@@ -258,10 +311,18 @@ def main() -> None:
     s, kw, B, T, alg_bytes = _make(a.workload)
     A = Sampler(s, B, "cuda:0", seed=0, **kw)
     A.set_temperature(T)
-    S = A.init_spins(seed=0)
-    S0 = S.cpu().numpy()
+    if a.workload == "xy":
+        S = device_xy_spins(A, seed=0)
+    else:
+        S = A.init_spins(seed=0)
+    S0 = S[:16].cpu().numpy()                  # the replicas (words) the CPU baseline runs
     ncls = len(A.classes)
     upd_per_sweep = B * s.n_active
+    resident = A.module.resident is not None and A.use_resident
+    # a resident launch runs `per` whole sweeps; otherwise one launch is one class of one sweep
+    per = 10 if resident else a.steps
+    nl = max(1, a.steps // per)
+    steps = nl * per
     A.sweep(S, a.warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -269,31 +330,34 @@ def main() -> None:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    A.sweep(S, a.steps)
+    for _ in range(nl):
+        A.sweep(S, per)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
     sec = ms / 1e3
-    value = upd_per_sweep * a.steps / sec
-    # dominant kernel: the class launches; the counter bump (one thread per sweep) is negligible
-    launch_s = sec / (a.steps * ncls)
-    bytes_per_launch = upd_per_sweep / ncls * alg_bytes
+    value = upd_per_sweep * steps / sec
+    # dominant kernel: the class launches (or the resident kernel); the counter bump is negligible
+    launches = nl if resident else steps * ncls
+    launch_s = sec / launches
+    bytes_per_launch = upd_per_sweep * steps / launches * alg_bytes
     achieved = bytes_per_launch / launch_s / 1e9
     out = {
         "metric": "site updates/s (%HBM roofline)",
         "value": value,
         "unit": "site updates/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": ms / a.steps,
+        "ms_per_step": ms / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bitwise": "u32", "ising": "int8", "heisenberg": "f32", "xy": "f32"}[a.workload],
         "data": "synthetic (random initial spins from the seeded torch CPU generator)",
         "config": {"workload": WORKLOADS[a.workload], "batch": B, "sites": s.n_active, "classes": ncls,
+                   "kernel": "sck_resident" if resident else "sck_class_*", "layout": A.layout,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload),

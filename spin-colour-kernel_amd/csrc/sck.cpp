@@ -434,16 +434,17 @@ int sck_sample_sites(sck_handle *h, const sck_args *args, const int32_t *rep_sit
 int sck_time_replays(sck_graph *g, int32_t reps, void *stream, float *ms) {
     if (!g || reps <= 0 || !ms) return fail(SCK_E_INVALID, "invalid timing arguments");
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
-    HIP_TRY(hipEventRecord(a, s));
-    for (int32_t i = 0; i < reps; ++i) HIP_TRY(hipGraphLaunch(g->exec, s));
-    HIP_TRY(hipEventRecord(b, s));
-    HIP_TRY(hipEventSynchronize(b));
-    HIP_TRY(hipEventElapsedTime(ms, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    for (int32_t i = 0; i < reps && e == hipSuccess; ++i) e = hipGraphLaunch(g->exec, s);
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (e != hipSuccess) return fail(SCK_E_HIP, std::string("replay timing: ") + hipGetErrorString(e));
     return SCK_OK;
 }
 

@@ -48,6 +48,7 @@ __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT
 
 MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP = range(5)
 BLOCK = 256
+XCDS = 8                      # XCDs of an MI355X; consecutive workgroups are dispatched to them round-robin
 WORD = 32
 GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
 LEVELS = 32
@@ -272,7 +273,8 @@ DEV int wrap_add(int v, int d, int L) {
 
 
 class _Gen:
-    def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False):
+    def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False,
+                 rm_tile=None):
         self.s = sys_
         self.mode = mode
         self.B = int(B)
@@ -295,6 +297,7 @@ class _Gen:
         # replica-major storage [b][L3][L2][L1][n][B] (lanes over replicas) or the reference layout
         self.replica_major = bool(replica_major) and mode in (MODE_INT8, MODE_F32)
         self.SS = self.B if self.replica_major else 1
+        self.rm_tile = self.default_rm_tile() if rm_tile is None else tuple(int(v) for v in rm_tile)
         self.rt = False               # box dimensions are compile-time constants (runtime: batch kernels)
         self.lines: list[str] = []
 
@@ -684,20 +687,50 @@ class _Gen:
             w("}")
             w()
 
+    def default_rm_tile(self):
+        """Class slots per tile (x slots, rows, planes) of the replica-major kernels: a tile's sites and most of
+        their neighbours stay within the L2 window of the workgroups one XCD holds in flight."""
+        if self.L3 > 1:
+            return (min(self.BX, 8), min(self.L2, 8), min(self.L3, 4))
+        return (min(self.BX, 32), min(self.L2, 16), 1)
+
+    def rm_plan(self):
+        """(groups, slots per group, items per XCD) of the replica-major class kernels; slots are padded to
+        whole tiles."""
+        tx, ty, tz = self.rm_tile
+        ntile = -(-self.BX // tx) * -(-self.L2 // ty) * -(-self.L3 // tz)
+        S = ntile * tx * ty * tz
+        G = -(-self.B // (BLOCK * self.rm_chunks()))
+        return G, S, -(-(G * S) // XCDS)
+
     def emit_rm_class(self, k: int, c: int, a: int, T: str) -> None:
         """Replica-major class kernel: the 64 lanes of a wave are 64 consecutive replicas of one class site, so
         the site, its colour residue and every neighbour index are wave-uniform (scalar) and each spin load is
-        one coalesced 256-byte access. blockIdx.x is the replica group (fastest, so a group stays on one XCD
-        and its L2), blockIdx.y/z the class slot."""
+        one coalesced 256-byte access.
+
+        Work items are (replica group, class slot) pairs. Workgroups are dispatched round-robin over the 8 XCDs,
+        so blockIdx.x is the XCD and (blockIdx.y, blockIdx.z) the position inside that XCD's contiguous share
+        of the items; slots run through 3-D tiles (x slots fastest, then rows, then planes), so a site's
+        neighbours are mostly visited by workgroups of the same XCD shortly before or after it and come from
+        that XCD's L2."""
         w = self.w
         R = self.rm_chunks()
+        G, S, chunk = self.rm_plan()
+        tx, ty, tz = self.rm_tile
+        ntx, nty = -(-self.BX // tx), -(-self.L2 // ty)
+        tsz = tx * ty * tz
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
-        w(f"    const int rep0 = blockIdx.x * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
-        w("    const int t = blockIdx.y + blockIdx.z * gridDim.y;")
-        w(f"    const int row = t / {self.BX};")
-        w(f"    if (row >= {self.ROWS}) return;")
-        w(f"    const int j = t - row * {self.BX};")
-        w(f"    const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+        w(f"    const int64_t v = (int64_t)blockIdx.x * {chunk} + blockIdx.y + (int64_t)blockIdx.z * gridDim.y;")
+        w(f"    if (blockIdx.y + (int64_t)blockIdx.z * gridDim.y >= {chunk} || v >= {G * S}LL) return;")
+        w(f"    const int grp = (int)(v / {S}), t = (int)(v - (int64_t)grp * {S});")
+        w(f"    const int rep0 = grp * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
+        w(f"    const int tt = t / {tsz}, u = t - tt * {tsz};")
+        w(f"    const int tj = tt % {ntx}, tr = tt / {ntx};")
+        w(f"    const int uy = (u / {tx}) % {ty}, uz = u / {tx * ty};")
+        w(f"    const int j = tj * {tx} + (u % {tx});")
+        w(f"    const int y = (tr % {nty}) * {ty} + uy, z = (tr / {nty}) * {tz} + uz;")
+        w(f"    if (j >= {self.BX} || y >= {self.L2} || z >= {self.L3}) return;")
+        w(f"    const int row = z * {self.L2} + y;")
         self.emit_residue(c, a)
         w(f"    const int x = x0 + {self.P} * j;")
         w(f"    if (x >= {self.L1}) return;")
@@ -1460,9 +1493,9 @@ class _Gen:
                 tile = BLOCK * self.bw_rounds(self.classes[k][1])
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // tile), self.W, 1)))
             elif self.replica_major:
-                slots = self.ROWS * self.BX
-                gy = min(slots, 65535)
-                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.B // (BLOCK * self.rm_chunks())), gy, -(-slots // gy))))
+                _, _, chunk = self.rm_plan()
+                gy = min(chunk, 65535)
+                sweep.append(Launch(ki[f"sck_class_{k}"], (XCDS, gy, -(-chunk // gy))))
             else:
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // BLOCK), gy, gz)))
         ref = [Launch(ki["sck_seq"], (-(-R // 64), 1, 1), (64, 1, 1))]
@@ -1702,11 +1735,11 @@ def fits_resident(sys_: System, mode: int) -> bool:
 
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
-             replica_major: bool = False) -> Module:
+             replica_major: bool = False, rm_tile=None) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major).generate()
+    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile).generate()

@@ -88,7 +88,8 @@ class Sampler:
                  spin_dtype: torch.dtype | None = None, packed: bool | None = None, bitwise: bool = False,
                  words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
                  bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
-                 width=None, learned=None, cache_dir=None, resident: bool = True, layout: str = "auto"):
+                 width=None, learned=None, cache_dir=None, resident: bool = True, layout: str = "auto",
+                 tile=None):
         self.sys = system
         self.use_resident = bool(resident)
         self.bitwise = bool(bitwise)
@@ -156,7 +157,7 @@ class Sampler:
         self.replica_major = layout == "replica"
         self.layout = layout
         self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta,
-                                 replica_major=self.replica_major)
+                                 replica_major=self.replica_major, rm_tile=tile)
         self.src = self.module.src
         dev = self.device
         m = system.site_mask()

@@ -1,0 +1,88 @@
+"""Distribution-level validation against exact enumeration (the reference's experiment E2).
+
+CPU: the enumeration against the known density of states of the 4 x 4 periodic Ising model and against the
+closed form of an open two-site chain in a field; bin merging and Holm's correction.
+GPU: sampled energies of the coloured sweep pass a chi-square test against the exact distribution (int8,
+packed and bitwise storage, Barker and Metropolis), and the synchronous all-sites control is rejected.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from sck_amd import crystals as cr
+from sck_amd import system as sy
+from sck_amd.validate import energy_test, exact_energy_distribution, holm, merge_bins
+
+DEV = "cuda"
+
+
+def test_square_4x4_density_of_states():
+    ex = exact_energy_distribution(sy.ising("square", 4), 2.269)
+    got = dict(zip(ex["levels"].astype(int).tolist(), ex["degeneracy"].tolist()))
+    # 2^16 states: g(-32) = 2, g(-24) = 32, g(-20) = 64, g(-16) = 424, g(-12) = 1728, g(-8) = 6688,
+    # g(-4) = 13568, g(0) = 20524, symmetric under E -> -E
+    half = {-32: 2, -24: 32, -20: 64, -16: 424, -12: 1728, -8: 6688, -4: 13568}
+    want = dict(half)
+    want.update({-k: v for k, v in half.items()})
+    want[0] = 20524
+    assert got == want and sum(got.values()) == 2 ** 16
+
+
+def test_two_spins_in_a_field_closed_form():
+    s = sy.System("pair", 1, cr.make("sc"), (2, 1, 1), (False,) * 3, {1: 1.5}, h=(0.4,))
+    T = 1.3
+    ex = exact_energy_distribution(s, T)
+    # states: ++ : -J - 2h, -- : -J + 2h, +- and -+ : +J
+    E = {round(-1.5 - 0.8, 6): 1, round(-1.5 + 0.8, 6): 1, 1.5: 2}
+    Z = sum(g * math.exp(-e / T) for e, g in E.items())
+    for lv, p in zip(ex["levels"], ex["p"]):
+        assert abs(p - E[round(float(lv), 6)] * math.exp(-lv / T) / Z) < 1e-12
+
+
+def test_bin_merging_and_holm():
+    o, e = merge_bins(np.array([1, 2, 10, 3, 1.0]), np.array([3, 3, 9, 2, 4.0]))
+    assert np.array_equal(e, [6.0, 9.0, 6.0]) and np.array_equal(o, [3.0, 10.0, 4.0])
+    o, e = merge_bins(np.array([1, 2, 10, 3, 1.0]), np.array([3, 3, 9, 2, 1.0]))      # short tail joins the last bin
+    assert np.array_equal(e, [6.0, 12.0]) and np.array_equal(o, [3.0, 14.0])
+    assert np.allclose(holm([0.01, 0.04, 0.03]), [0.03, 0.06, 0.06])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["int8-barker", "packed-barker", "bitwise-barker", "int8-metropolis-tri"])
+def test_coloured_sweep_samples_the_exact_distribution(case):
+    from sck_amd.sampler import Sampler
+    if case.endswith("tri"):
+        s, T = sy.ising("triangular", 3), 4.0 / math.log(3.0)
+    else:
+        s, T = sy.ising("square", 4), 2.0 / math.log(1 + math.sqrt(2))
+        s.acceptance = "barker"
+    kw = {"int8": dict(packed=False), "packed": dict(packed=True), "bitwise": dict(bitwise=True, independent=True)}[
+        case.split("-")[0]]
+    print(f"[gpu] exact distribution {case}", flush=True)
+    r = energy_test(lambda seed: Sampler(s, 1024, DEV, seed=seed, **kw), s, T, seeds=(1, 2, 3), samples=30000,
+                    batch=1024)
+    assert all(row["off_level"] == 0 for row in r["rows"])
+    assert r["rejected"] == 0, r["rows"]
+
+
+@pytest.mark.gpu
+def test_synchronous_control_is_rejected():
+    """The all-sites update is not a valid sampler of the Boltzmann distribution: the test must see it."""
+    from sck_amd.sampler import Sampler
+    s, T = sy.ising("square", 4), 2.0 / math.log(1 + math.sqrt(2))
+    s.acceptance = "barker"
+    r = energy_test(lambda seed: Sampler(s, 1024, DEV, seed=seed, packed=False), s, T, seeds=(1, 2, 3),
+                    samples=30000, batch=1024, rule="all sites")
+    assert r["rejected"] == 3, r["rows"]
+
+
+@pytest.mark.gpu
+def test_rate_of_graph_replays():
+    from sck_amd.perf import boot_ci, rate
+    r, info = rate(sy.ising("square", 32), 256, DEV, T=2.5, reps=5)
+    assert len(r) == 5 and min(r) > 0 and info["sweeps_per_replay"] == math.ceil(2e7 / (256 * 1024))
+    m, lo, hi = boot_ci(r)
+    assert lo <= m <= hi
