@@ -36,7 +36,7 @@ WORKLOADS = {
     "heisenberg": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, fp32, B=8192 (32 temps x 256 in [0.5,3.0])",
     "xy": "XY triangular L=2048 periodic J1=1, fp32, B=2048 (16 temps x 128 in [0.7,1.1]), replica-major",
     "learned": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, B=8192 (32 temps in [0.5,3.0]): effective "
-               "samples/s of the learned cone width over the best fixed width",
+               "samples/s of the learned cone width over the best of five fixed widths (by measured ESS/s)",
     "nanodisks": "64 Heisenberg sc nanodisks, radius U[8,40], thickness U{4..12} (numpy default_rng(2024)), open, "
                  "J1=1, bulk DMI D=0.1, Ks=0.3 along the surface normal, 16 replicas each (T in [0.3,1.8]), fp32, "
                  "every disk in one launch per class, graph-captured sweeps, autotuned schedule",
@@ -185,7 +185,7 @@ def learned_main(a) -> None:
            "config": {"workload": WORKLOADS["learned"], "batch": B, "sites": s.n_active},
            "gain_E": r["gain_E"], "gain_M": r["gain_M"], "gain_E_bottleneck": r["gain_E_bottleneck"],
            "gain_M_bottleneck": r["gain_M_bottleneck"], "gain_E_sum": r["gain_E_sum"], "gain_M_sum": r["gain_M_sum"],
-           "fixed_width": r["fixed_width"],
+           "fixed_width": r["fixed_width"], "fixed_scan_gm_ess_per_s": r["fixed_scan_gm_ess"],
            "vs_uniform_redraw": {k: r["vs_uniform"][k] for k in ("gain_E", "gain_M", "gain_E_bottleneck",
                                                                    "gain_M_bottleneck")},
            "ess_per_s": {k: {"E": r[k]["ess_E_per_s"], "M": r[k]["ess_M_per_s"],

@@ -2,9 +2,9 @@
 
 Three proposals run from one common equilibrium state (equilibrated with the learned width, the fastest-mixing
 chain; every proposal keeps detailed balance, so the state stays stationary under each): the uniform redraw
-(the reference's proposal, SCK/generator.py:281-289), the best single fixed half-angle (the largest training
-objective on the warm-up records: the learned network's own objective restricted to one constant, a tuned
-fixed-width Metropolis) and the learned width. tau_int comes from Sokal's window on the energy and |M| series
+(the reference's proposal, SCK/generator.py:281-289), a scan of fixed half-angles, and the learned width. The
+fixed-width baseline is the scanned width with the best measured ESS/s (geometric mean over the ladder), a
+tuned fixed-width Metropolis. tau_int comes from Sokal's window on the energy and |M| series
 of each temperature (sck_amd.observe). Reported gains of one proposal over another: per temperature, the
 ratio of ESS/s; their geometric mean over the ladder (the headline); the bottleneck ratio (slowest
 temperature against slowest temperature); and the ratio of the summed ESS/s, which the fastest (hottest)
@@ -65,10 +65,12 @@ def _zscores(a: dict, b: dict) -> np.ndarray:
 
 
 def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 1000, therm: int = 50,
-                      n_meas: int = 1000, every: int = 1, seed: int = 0, train_steps: int = 400, log=print) -> dict:
+                      n_meas: int = 1000, every: int = 1, seed: int = 0, train_steps: int = 400, log=print,
+                      widths=(0.5, 1.0, 1.5, 2.0, 2.5)) -> dict:
     """Train the width on `warmup` fixed-width sweeps, equilibrate one state for `equil` sweeps with the
     learned width, then from that common equilibrium state run `therm` + `n_meas` sweeps of each of: the
-    uniform redraw (the reference's proposal), the best single fixed width, and the learned width."""
+    uniform redraw (the reference's proposal), each fixed width of `widths`, and the learned width. The
+    fixed baseline is the width with the largest geometric-mean ESS/s of E over the ladder, measured."""
     temps = np.asarray(temps, np.float32)
     per = B // len(temps)
     T_rep = np.repeat(temps, per)
@@ -92,8 +94,9 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     Q.set_temperature(torch.as_tensor(T_rep))
     state = S0.clone()
     Q.sweep(state, equil)
-    kinds = (("uniform", dict(proposal="uniform")), ("fixed", dict(proposal="cone", width=w_fixed)),
-             ("learned", dict(proposal="learned", learned=lw)))
+    kinds = [("uniform", dict(proposal="uniform"))] + \
+        [(f"fixed_{w:g}", dict(proposal="cone", width=float(w))) for w in widths] + \
+        [("learned", dict(proposal="learned", learned=lw))]
     for label, kw in kinds:
         A = Sampler(system, B, "cuda", seed=seed + 1, **kw)
         A.set_temperature(torch.as_tensor(T_rep))
@@ -109,6 +112,16 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
         log(f"  {label:8s}: {secs:7.2f} s, ESS/s (E) {essE / secs:10.1f}, ESS/s (|M|) {essM / secs:10.1f}, "
             f"updates/s {B * N * n_meas * every / secs:.3e}")
     nt = len(temps)
+    # the fixed baseline: the measured-best constant width (geometric mean over the ladder of ESS/s of E)
+    def gm(r):
+        return float(np.exp(np.mean([np.log(1.0 / (r["tau_E"][j] * r["seconds"])) for j in range(nt)])))
+    fixed_labels = [f"fixed_{w:g}" for w in widths]
+    best = max(fixed_labels, key=lambda k: gm(result[k]))
+    result["fixed"] = result[best]
+    result["fixed_width"] = float(best.split("_", 1)[1])
+    result["fixed_width_by_objective"] = w_fixed
+    result["fixed_scan_gm_ess"] = {k: gm(result[k]) for k in fixed_labels + ["uniform", "learned"]}
+    log(f"  best fixed width by measured ESS/s: {result['fixed_width']}")
     result.update(_gains(result["fixed"], result["learned"], nt))
     result["vs_uniform"] = _gains(result["uniform"], result["learned"], nt)
     result["fixed_vs_uniform"] = _gains(result["uniform"], result["fixed"], nt)

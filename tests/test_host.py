@@ -140,3 +140,36 @@ def test_sampler_refuses_cpu_device():
     from sck_amd.sampler import Sampler
     with pytest.raises(ValueError, match="HIP device"):
         Sampler(sy.ising("square", 4), 2, device="cpu")
+
+
+@pytest.mark.parametrize("case", ["bcc32-8192", "tri64-2048", "sq30-192-tile", "sc10-320-tile"])
+def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
+    """The launch plan's grid with the kernel's decode (XCD share -> (replica group, tiled slot)) visits every
+    (group, class slot) exactly once; ragged tiles and the padded XCD shares fall out of range."""
+    from sck_amd import hipgen as G
+    mk = {"bcc32-8192": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
+                                            {1: 1.0, 2: -0.2}), 8192, None),
+          "tri64-2048": (lambda: sy.xy("triangular", 64), 2048, None),
+          "sq30-192-tile": (lambda: sy.xy("square", 30), 192, (7, 5, 1)),
+          "sc10-320-tile": (lambda: sy.heisenberg("sc", 10), 320, (3, 4, 3))}[case]
+    s, B, tile = mk[0](), mk[1], mk[2]
+    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile)
+    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile)
+    Gn, S, chunk = g.rm_plan()
+    tx, ty, tz = g.rm_tile
+    ntx, nty = -(-g.BX // tx), -(-g.L2 // ty)
+    gx, gy, gz = mod.sweep[0].grid
+    assert gx == G.XCDS and gy * gz >= chunk
+    seen = np.zeros((Gn, g.L3, g.L2, g.BX), np.int64)
+    for bx in range(gx):
+        k = np.arange(gy * gz)
+        v = bx * chunk + k[k < chunk]
+        v = v[v < Gn * S]
+        grp, t = v // S, v % S
+        tt, u = t // (tx * ty * tz), t % (tx * ty * tz)
+        j = (tt % ntx) * tx + u % tx
+        y = ((tt // ntx) % nty) * ty + (u // tx) % ty
+        z = ((tt // ntx) // nty) * tz + u // (tx * ty)
+        ok = (j < g.BX) & (y < g.L2) & (z < g.L3)
+        np.add.at(seen, (grp[ok], z[ok], y[ok], j[ok]), 1)
+    assert seen.min() == 1 and seen.max() == 1
