@@ -51,16 +51,21 @@ def test_bin_merging_and_holm():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["int8-barker", "packed-barker", "bitwise-barker", "int8-metropolis-tri"])
+@pytest.mark.parametrize("case", ["int8-barker-sq4", "packed-barker-sq4", "bitwise-barker-sq4", "int8-metropolis-sq4",
+                                  "int8-barker-tri3"])
 def test_coloured_sweep_samples_the_exact_distribution(case):
+    """Metropolis on the 3 x 3 triangular lattice is left out on purpose: with the deterministic class scan every
+    dE = 0 flip is certain and the chain is not ergodic there (the sequential oracle, the reference's chain,
+    shows the same bias); the heat-bath (Barker) rule samples it."""
     from sck_amd.sampler import Sampler
-    if case.endswith("tri"):
+    store, acc, lat = case.split("-")
+    if lat == "tri3":
         s, T = sy.ising("triangular", 3), 4.0 / math.log(3.0)
     else:
         s, T = sy.ising("square", 4), 2.0 / math.log(1 + math.sqrt(2))
-        s.acceptance = "barker"
+    s.acceptance = acc
     kw = {"int8": dict(packed=False), "packed": dict(packed=True), "bitwise": dict(bitwise=True, independent=True)}[
-        case.split("-")[0]]
+        store]
     print(f"[gpu] exact distribution {case}", flush=True)
     r = energy_test(lambda seed: Sampler(s, 1024, DEV, seed=seed, **kw), s, T, seeds=(1, 2, 3), samples=30000,
                     batch=1024)
