@@ -298,6 +298,9 @@ class _Gen:
         self.replica_major = bool(replica_major) and mode in (MODE_INT8, MODE_F32)
         self.SS = self.B if self.replica_major else 1
         self.rm_tile = self.default_rm_tile() if rm_tile is None else tuple(int(v) for v in rm_tile)
+        esz = 1 if mode == MODE_INT8 else 4
+        # one replica's spins addressed by a 32-bit byte offset from its base pointer
+        self.u32 = mode in (MODE_INT8, MODE_F32) and self.n * self.NS * esz < 2 ** 31
         self.rt = False               # box dimensions are compile-time constants (runtime: batch kernels)
         self.lines: list[str] = []
 
@@ -307,10 +310,46 @@ class _Gen:
         [site][n][B]: a wave-uniform byte address of the site (scalar) plus the lane's 32-bit offset
         (replica and component), so each load takes a scalar base and a vector offset."""
         if self.replica_major:
+            return f"rm_load({ptr}, (int64_t)({site}), (uint32_t)rbase, {comp})"
+        if self.u32:
+            # the replica's base pointer (wave-uniform wherever the replica is) plus a 32-bit byte offset
             esz = 1 if self.mode == MODE_INT8 else 4
-            return (f"(*(decltype({ptr}))((const char *)({ptr}) + (int64_t)({site}) * {self.n * self.B * esz} + "
-                    f"((uint32_t)rbase + {comp * self.B}u) * {esz}u))")
+            return (f"(*(decltype({ptr}))((const char *)({ptr} + rbase) + "
+                    f"(uint32_t)({self._mul(comp, self.NS)} + {site}) * {esz}u))")
         return f"{ptr}[rbase + (int64_t)({self._mul(comp, self.NS)} + {site})]"
+
+    def store(self, ptr: str, comp: int, site: str, value: str) -> str:
+        """Statement writing `value` to component `comp` of site `site` of the replica at `rbase`."""
+        if self.replica_major:
+            return f"rm_store({ptr}, (int64_t)({site}), (uint32_t)rbase, {comp}, {value});"
+        return f"{self.at(ptr, comp, site)} = {value};"
+
+    def emit_rm_access(self) -> None:
+        """Replica-major loads and stores as buffer instructions: one wave-uniform buffer resource per site
+        (scalar registers), the component's offset as the scalar offset and the lane's replica as the only
+        vector offset, so an access costs no vector address arithmetic."""
+        w = self.w
+        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        esz = 1 if self.mode == MODE_INT8 else 4
+        w(f"DEV __amdgpu_buffer_rsrc_t rm_site(const {T} *base, int64_t site) {{")
+        w(f"    return __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)base + site * {self.n * self.B * esz}), "
+          f"0, 0x7fffffff, 0x00020000);")
+        w("}")
+        w(f"DEV {T} rm_load(const {T} *base, int64_t site, uint32_t rep, int comp) {{")
+        if esz == 4:
+            w(f"    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rm_site(base, site), rep * 4u, "
+              f"comp * {self.B * 4}, 0));")
+        else:
+            w(f"    return (int8_t)__builtin_amdgcn_raw_buffer_load_b8(rm_site(base, site), rep, comp * {self.B}, 0);")
+        w("}")
+        w(f"DEV void rm_store({T} *base, int64_t site, uint32_t rep, int comp, {T} v) {{")
+        if esz == 4:
+            w(f"    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rm_site(base, site), rep * 4u, "
+              f"comp * {self.B * 4}, 0);")
+        else:
+            w(f"    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, rm_site(base, site), rep, comp * {self.B}, 0);")
+        w("}")
+        w()
 
     def rbase_expr(self, rep: str) -> str:
         """Offset of replica `rep`'s first spin: the address of component k of site q is
@@ -440,7 +479,7 @@ class _Gen:
             self.emit_draw("        ")
             w(f"        acc = {self.accept_expr('p')};")
             w("    }")
-            w(f"    if (acc) {self.at('dst', 0, 'own')} = ({T})(-s0);")
+            w(f"    if (acc) {{ {self.store('dst', 0, 'own', f'({T})(-s0)')} }}")
             w("    return acc;")
             w("}")
             return
@@ -483,7 +522,7 @@ class _Gen:
             w("    const int acc = u0 < exp_fast((-beta) * dE);")
         w("    if (acc) {")
         for k in range(n):
-            w(f"        {self.at('dst', k, 'own')} = t{k};")
+            w(f"        {self.store('dst', k, 'own', f't{k}')}")
         w("    }")
         w("    return acc;")
         w("}")
@@ -609,6 +648,8 @@ class _Gen:
     def emit_plain(self) -> None:
         w = self.w
         T = "int8_t" if self.mode == MODE_INT8 else "float"
+        if self.replica_major:
+            self.emit_rm_access()
         for a in range(self.b):
             self.emit_update_plain(a)
             w()
@@ -1551,6 +1592,8 @@ class _BatchGen(_Gen):
 
     def __init__(self, systems, mode, proposal, learned_meta, wide):
         super().__init__(systems[0], mode, 1, proposal, learned_meta, replica_major=False)
+        esz = 1 if mode == MODE_INT8 else 4
+        self.u32 = max(s.n * s.nsite for s in systems) * esz < 2 ** 31
         self.rt = True
         self.L1, self.L2, self.L3, self.PL, self.NS = "D.L1", "D.L2", "D.L3", "D.PL", "D.NS"
         self.wide = bool(wide)
