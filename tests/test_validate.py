@@ -91,3 +91,37 @@ def test_rate_of_graph_replays():
     assert len(r) == 5 and min(r) > 0 and info["sweeps_per_replay"] == math.ceil(2e7 / (256 * 1024))
     m, lo, hi = boot_ci(r)
     assert lo <= m <= hi
+
+
+# ------------------------------------------------------------------------------ application observables
+def test_topological_charge_of_uniform_and_skyrmion_states():
+    from sck_amd.observables import topological_charge
+    L = 41
+    s = sy.System("heis square", 3, cr.make("square"), (L, L), (False, False), {1: 1.0})
+    yy, xx = np.meshgrid(np.arange(L) - L // 2, np.arange(L) - L // 2, indexing="ij")
+    lam = 5.0
+    r2 = xx ** 2 + yy ** 2
+    # Belavin-Polyakov skyrmion: m = (2 lam x, 2 lam y, lam^2 - r^2) / (lam^2 + r^2), m_z = +1 at the core
+    sk = np.stack([2 * lam * xx, 2 * lam * yy, lam ** 2 - r2]) / (lam ** 2 + r2)
+    up = np.zeros_like(sk)
+    up[2] = 1.0
+    S = torch.as_tensor(np.stack([up, sk, -sk])[:, :, None, None], dtype=torch.float32)   # (3, 3, 1, 1, L, L)
+    Q = topological_charge(S, s).numpy()
+    assert abs(Q[0]) < 1e-12
+    # the lattice charge of a finite box misses the tail beyond it: |Q| = 1 - lam^2 / (lam^2 + R^2) roughly
+    assert 0.9 < abs(Q[1]) < 1.0 and abs(Q[1] + Q[2]) < 1e-9
+
+
+def test_structure_factor_of_the_120_degree_state():
+    from sck_amd.observables import site_positions, structure_factor
+    L = 12
+    s = sy.xy("triangular", L)
+    K = np.array([4 * np.pi / 3, 0.0])
+    pos = site_positions(s)
+    th = pos @ np.pad(K, (0, 1))                                   # the 120 degree state: angle K . r
+    S = torch.as_tensor(np.stack([np.cos(th), np.sin(th)])[None], dtype=torch.float64)
+    sk = structure_factor(S, s, K).numpy()
+    assert abs(sk[0] - L * L / 2) < 1e-9
+    ferro = torch.zeros_like(S)
+    ferro[:, 0] = 1.0
+    assert abs(structure_factor(ferro, s, [0.0, 0.0]).numpy()[0] - L * L) < 1e-9
