@@ -274,7 +274,7 @@ DEV int wrap_add(int v, int d, int L) {
 
 class _Gen:
     def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False,
-                 rm_tile=None):
+                 rm_tile=None, rm_chunks=None):
         self.s = sys_
         self.mode = mode
         self.B = int(B)
@@ -298,6 +298,7 @@ class _Gen:
         self.replica_major = bool(replica_major) and mode in (MODE_INT8, MODE_F32)
         self.SS = self.B if self.replica_major else 1
         self.rm_tile = self.default_rm_tile() if rm_tile is None else tuple(int(v) for v in rm_tile)
+        self.rm_R = None if rm_chunks is None else max(1, int(rm_chunks))
         esz = 1 if mode == MODE_INT8 else 4
         # one replica's spins addressed by a 32-bit byte offset from its base pointer
         self.u32 = mode in (MODE_INT8, MODE_F32) and self.n * self.NS * esz < 2 ** 31
@@ -824,7 +825,10 @@ class _Gen:
         w()
 
     def rm_chunks(self) -> int:
-        """64-replica chunks one wave of a replica-major class kernel runs for its site."""
+        """64-replica chunks one wave of a replica-major class kernel runs for its site: more chunks share the
+        site's scalar work, fewer keep the workgroups in flight on an XCD within its L2."""
+        if self.rm_R is not None:
+            return self.rm_R
         return max(1, min(4, self.B // BLOCK))
 
     def emit_residue(self, c: int, a: int) -> None:
@@ -1778,11 +1782,11 @@ def fits_resident(sys_: System, mode: int) -> bool:
 
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
-             replica_major: bool = False, rm_tile=None) -> Module:
+             replica_major: bool = False, rm_tile=None, rm_chunks=None) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile).generate()
+    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks).generate()

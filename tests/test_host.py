@@ -142,7 +142,8 @@ def test_sampler_refuses_cpu_device():
         Sampler(sy.ising("square", 4), 2, device="cpu")
 
 
-@pytest.mark.parametrize("case", ["bcc32-8192", "tri64-2048", "sq30-192-tile", "sc10-320-tile"])
+@pytest.mark.parametrize("case", ["bcc32-8192", "tri64-2048", "sq30-192-tile", "sc10-320-tile", "bcc32-8192-R1",
+                                  "bcc32-8192-R2"])
 def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
     """The launch plan's grid with the kernel's decode (XCD share -> (replica group, tiled slot)) visits every
     (group, class slot) exactly once; ragged tiles and the padded XCD shares fall out of range."""
@@ -151,10 +152,15 @@ def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
                                             {1: 1.0, 2: -0.2}), 8192, None),
           "tri64-2048": (lambda: sy.xy("triangular", 64), 2048, None),
           "sq30-192-tile": (lambda: sy.xy("square", 30), 192, (7, 5, 1)),
-          "sc10-320-tile": (lambda: sy.heisenberg("sc", 10), 320, (3, 4, 3))}[case]
+          "sc10-320-tile": (lambda: sy.heisenberg("sc", 10), 320, (3, 4, 3)),
+          "bcc32-8192-R1": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
+                                              {1: 1.0, 2: -0.2}), 8192, (4, 4, 4), 1),
+          "bcc32-8192-R2": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
+                                              {1: 1.0, 2: -0.2}), 8192, (8, 4, 2), 2)}[case]
     s, B, tile = mk[0](), mk[1], mk[2]
-    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile)
-    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile)
+    R = mk[3] if len(mk) > 3 else None
+    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile, rm_chunks=R)
+    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile, rm_chunks=R)
     Gn, S, chunk = g.rm_plan()
     tx, ty, tz = g.rm_tile
     ntx, nty = -(-g.BX // tx), -(-g.L2 // ty)
