@@ -51,6 +51,7 @@ BLOCK = 256
 XCDS = 8                      # XCDs of an MI355X; consecutive workgroups are dispatched to them round-robin
 WORD = 32
 GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
+BW_QUEUE_DIV = 4              # independent bitwise mode: the tail queue holds 1/4 of a block's items
 LEVELS = 32
 
 
@@ -1104,8 +1105,9 @@ class _Gen:
         nd = self.bw_ndyn[a]
         item = 3 + 2 * max(1, nd)
         R = self.bw_rounds(a)
+        qcap = max(64, R * BLOCK // BW_QUEUE_DIV)
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
-        w(f"    __shared__ uint32_t q[{R * BLOCK}][{item}];")
+        w(f"    __shared__ uint32_t q[{qcap}][{item}];")
         w("    __shared__ int qn;")
         w("    if (threadIdx.x == 0) qn = 0;")
         w("    __syncthreads();")
@@ -1142,14 +1144,20 @@ class _Gen:
         w("            qb = __shfl(qb, 0, 64);")
         w("            if (tail) {")
         w("                const int pos = qb + __popcll(m & ((1ull << lane) - 1ull));")
-        w("                q[pos][0] = (uint32_t)own; q[pos][1] = st.acc; q[pos][2] = st.und;")
+        w(f"                if (pos < {qcap}) {{")
+        w("                    q[pos][0] = (uint32_t)own; q[pos][1] = st.acc; q[pos][2] = st.und;")
         for i in range(nd):
-            w(f"                q[pos][{3 + i}] = st.e[{i}]; q[pos][{3 + nd + i}] = st.P[{i}];")
+            w(f"                    q[pos][{3 + i}] = st.e[{i}]; q[pos][{3 + nd + i}] = st.P[{i}];")
+        w("                } else {")
+        w("                    // queue full: finish the item here, with the same level groups in the same order")
+        w(f"                    levels_{a}(st, ctr, key, 2, {LEVELS // 4});")
+        w("                    Sw[own] = s0 ^ st.acc;")
+        w("                }")
         w("            }")
         w("        }")
         w("    }")
         w("    __syncthreads();")
-        w("    const int n = qn;")
+        w(f"    const int n = qn < {qcap} ? qn : {qcap};")
         w(f"    for (int i = threadIdx.x; i < ((n + 63) & ~63); i += {BLOCK}) {{")
         w("        const bool has = i < n;")
         w(f"        BW{a} st;")
