@@ -282,6 +282,8 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="bitwise", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", default=None, help="replica-major kernels: class slots per tile, e.g. 8,8,4")
+    ap.add_argument("--chunks", type=int, default=None, help="replica-major kernels: 64-replica chunks per wave")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference for the HBM counters)")
     ap.add_argument("--schedule", default="auto",
@@ -309,6 +311,10 @@ def main() -> None:
         nanodisk_main(a)
         return
     s, kw, B, T, alg_bytes = _make(a.workload)
+    if a.tile:
+        kw["tile"] = tuple(int(v) for v in a.tile.split(","))
+    if a.chunks:
+        kw["chunks"] = a.chunks
     A = Sampler(s, B, "cuda:0", seed=0, **kw)
     A.set_temperature(T)
     if a.workload == "xy":
