@@ -135,28 +135,35 @@ def cpu_baseline(workload: str, s, A, S0, budget_s: float = 10.0) -> dict:
                                     f"sweeps, {dtn:.1f} s"}}
 
 
-def device_xy_spins(A, seed: int = 0):
-    """Random XY spins (uniform angles) drawn on the device one row block at a time with torch's seeded CUDA
-    generator, straight into the sampler's storage (a full-size batch is too large to draw on the host)."""
+def device_spins(A, seed: int = 0):
+    """Random XY (uniform angle) or Heisenberg (normalised Gaussian) spins drawn on the device one block at a
+    time with torch's seeded CUDA generator, straight into the sampler's storage (a full-size batch is too
+    large to draw quickly on the host)."""
     import math
 
     import torch
-    s = A.sys
+    s, n = A.sys, A.sys.n
     g = torch.Generator(device=A.device).manual_seed(int(seed))
+
+    def draw(shape_lead):                      # (..., n) unit vectors
+        if n == 2:
+            phi = torch.rand(shape_lead, generator=g, device=A.device) * (2 * math.pi)
+            return torch.stack([torch.cos(phi), torch.sin(phi)], dim=-1)
+        x = torch.randn(shape_lead + (n,), generator=g, device=A.device)
+        return x / x.norm(dim=-1, keepdim=True)
     if A.replica_major:
-        store = torch.empty((s.b, s.L3, s.L2, s.L1, 2, A.B), dtype=torch.float32, device=A.device)
-        for z in range(s.L3):
-            for y0 in range(0, s.L2, 64):
-                y1 = min(s.L2, y0 + 64)
-                phi = torch.rand((s.b, y1 - y0, s.L1, A.B), generator=g, device=A.device) * (2 * math.pi)
-                store[:, z, y0:y1, :, 0] = torch.cos(phi)
-                store[:, z, y0:y1, :, 1] = torch.sin(phi)
+        store = torch.empty((s.b, s.L3, s.L2, s.L1, n, A.B), dtype=torch.float32, device=A.device)
+        for a in range(s.b):
+            for z in range(s.L3):
+                for y0 in range(0, s.L2, 64):
+                    y1 = min(s.L2, y0 + 64)
+                    v = draw((y1 - y0, s.L1, A.B))                       # (rows, L1, B, n)
+                    store[a, z, y0:y1] = v.permute(0, 1, 3, 2)
         return store.permute(5, 4, 0, 1, 2, 3)
-    S = torch.empty((A.B, 2) + s.shape, dtype=torch.float32, device=A.device)
+    S = torch.empty((A.B, n) + s.shape, dtype=torch.float32, device=A.device)
     for r0 in range(0, A.B, 16):
-        phi = torch.rand((min(16, A.B - r0),) + s.shape, generator=g, device=A.device) * (2 * math.pi)
-        S[r0:r0 + 16, 0] = torch.cos(phi)
-        S[r0:r0 + 16, 1] = torch.sin(phi)
+        v = draw((min(16, A.B - r0),) + s.shape)                          # (reps, b, L3, L2, L1, n)
+        S[r0:r0 + 16] = v.permute(0, 5, 1, 2, 3, 4)
     return S
 
 
@@ -317,8 +324,8 @@ def main() -> None:
         kw["chunks"] = a.chunks
     A = Sampler(s, B, "cuda:0", seed=0, **kw)
     A.set_temperature(T)
-    if a.workload == "xy":
-        S = device_xy_spins(A, seed=0)
+    if a.workload in ("xy", "heisenberg"):
+        S = device_spins(A, seed=0)
     else:
         S = A.init_spins(seed=0)
     S0 = S[:16].cpu().numpy()                  # the replicas (words) the CPU baseline runs

@@ -275,7 +275,7 @@ DEV int wrap_add(int v, int d, int L) {
 
 class _Gen:
     def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False,
-                 rm_tile=None, rm_chunks=None):
+                 rm_tile=None, rm_chunks=None, rm_sites=None):
         self.s = sys_
         self.mode = mode
         self.B = int(B)
@@ -300,6 +300,10 @@ class _Gen:
         self.SS = self.B if self.replica_major else 1
         self.rm_tile = self.default_rm_tile() if rm_tile is None else tuple(int(v) for v in rm_tile)
         self.rm_R = None if rm_chunks is None else max(1, int(rm_chunks))
+        # class sites per workgroup: 1 (the 4 waves share a site, 256 R replicas) or 4 (a site per wave, 64 R)
+        self.rm_SP = 1 if rm_sites is None else int(rm_sites)
+        if self.rm_SP not in (1, BLOCK // 64):
+            raise ValueError(f"replica-major sites per workgroup must be 1 or {BLOCK // 64}")
         esz = 1 if mode == MODE_INT8 else 4
         # one replica's spins addressed by a 32-bit byte offset from its base pointer
         self.u32 = mode in (MODE_INT8, MODE_F32) and self.n * self.NS * esz < 2 ** 31
@@ -738,12 +742,12 @@ class _Gen:
         return (min(self.BX, 32), min(self.L2, 16), 1)
 
     def rm_plan(self):
-        """(groups, slots per group, items per XCD) of the replica-major class kernels; slots are padded to
-        whole tiles."""
+        """(groups, workgroup slots per group, items per XCD) of the replica-major class kernels; slots are
+        padded to whole tiles, and a workgroup takes rm_SP consecutive slots of one group."""
         tx, ty, tz = self.rm_tile
         ntile = -(-self.BX // tx) * -(-self.L2 // ty) * -(-self.L3 // tz)
-        S = ntile * tx * ty * tz
-        G = -(-self.B // (BLOCK * self.rm_chunks()))
+        S = -(-(ntile * tx * ty * tz) // self.rm_SP)
+        G = -(-self.B // (BLOCK * self.rm_chunks() // self.rm_SP))
         return G, S, -(-(G * S) // XCDS)
 
     def emit_rm_class(self, k: int, c: int, a: int, T: str) -> None:
@@ -765,8 +769,15 @@ class _Gen:
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
         w(f"    const int64_t v = (int64_t)blockIdx.x * {chunk} + blockIdx.y + (int64_t)blockIdx.z * gridDim.y;")
         w(f"    if (blockIdx.y + (int64_t)blockIdx.z * gridDim.y >= {chunk} || v >= {G * S}LL) return;")
-        w(f"    const int grp = (int)(v / {S}), t = (int)(v - (int64_t)grp * {S});")
-        w(f"    const int rep0 = grp * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
+        if self.rm_SP == 1:
+            w(f"    const int grp = (int)(v / {S}), t = (int)(v - (int64_t)grp * {S});")
+            w(f"    const int rep0 = grp * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
+        else:
+            # a site per wave: the wave index made scalar so the site arithmetic stays in scalar registers
+            w("    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);")
+            w(f"    const int grp = (int)(v / {S});")
+            w(f"    const int t = (int)(v - (int64_t)grp * {S}) * {self.rm_SP} + wave;")
+            w(f"    const int rep0 = grp * {64 * R} + (threadIdx.x & 63);")
         w(f"    const int tt = t / {tsz}, u = t - tt * {tsz};")
         w(f"    const int tj = tt % {ntx}, tr = tt / {ntx};")
         w(f"    const int uy = (u / {tx}) % {ty}, uz = u / {tx * ty};")
@@ -1790,11 +1801,11 @@ def fits_resident(sys_: System, mode: int) -> bool:
 
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
-             replica_major: bool = False, rm_tile=None, rm_chunks=None) -> Module:
+             replica_major: bool = False, rm_tile=None, rm_chunks=None, rm_sites=None) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks).generate()
+    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites).generate()

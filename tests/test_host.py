@@ -143,7 +143,7 @@ def test_sampler_refuses_cpu_device():
 
 
 @pytest.mark.parametrize("case", ["bcc32-8192", "tri64-2048", "sq30-192-tile", "sc10-320-tile", "bcc32-8192-R1",
-                                  "bcc32-8192-R2"])
+                                  "bcc32-8192-R2", "bcc32-8192-site-waves", "sq30-192-site-waves"])
 def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
     """The launch plan's grid with the kernel's decode (XCD share -> (replica group, tiled slot)) visits every
     (group, class slot) exactly once; ragged tiles and the padded XCD shares fall out of range."""
@@ -156,22 +156,32 @@ def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
           "bcc32-8192-R1": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
                                               {1: 1.0, 2: -0.2}), 8192, (4, 4, 4), 1),
           "bcc32-8192-R2": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
-                                              {1: 1.0, 2: -0.2}), 8192, (8, 4, 2), 2)}[case]
+                                              {1: 1.0, 2: -0.2}), 8192, (8, 4, 2), 2),
+          "bcc32-8192-site-waves": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
+                                                      {1: 1.0, 2: -0.2}), 8192, (4, 4, 4), 2, 4),
+          "sq30-192-site-waves": (lambda: sy.xy("square", 30), 192, (7, 5, 1), 1, 4)}[case]
     s, B, tile = mk[0](), mk[1], mk[2]
     R = mk[3] if len(mk) > 3 else None
-    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile, rm_chunks=R)
-    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile, rm_chunks=R)
+    SP = mk[4] if len(mk) > 4 else 1
+    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile, rm_chunks=R, rm_sites=SP)
+    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile, rm_chunks=R, rm_sites=SP)
     Gn, S, chunk = g.rm_plan()
     tx, ty, tz = g.rm_tile
     ntx, nty = -(-g.BX // tx), -(-g.L2 // ty)
     gx, gy, gz = mod.sweep[0].grid
     assert gx == G.XCDS and gy * gz >= chunk
+    # replicas of a group: every replica of the batch exactly once over the groups
+    Rc = g.rm_chunks()
+    per = 64 * Rc * (4 // SP)
+    reps = np.concatenate([grp * per + np.arange(per) for grp in range(Gn)])
+    assert np.array_equal(np.sort(reps[reps < B]), np.arange(B))
     seen = np.zeros((Gn, g.L3, g.L2, g.BX), np.int64)
     for bx in range(gx):
         k = np.arange(gy * gz)
         v = bx * chunk + k[k < chunk]
         v = v[v < Gn * S]
-        grp, t = v // S, v % S
+        grp = np.repeat(v // S, SP)
+        t = ((v % S)[:, None] * SP + np.arange(SP)[None, :]).reshape(-1) if SP > 1 else v % S
         tt, u = t // (tx * ty * tz), t % (tx * ty * tz)
         j = (tt % ntx) * tx + u % tx
         y = ((tt // ntx) % nty) * ty + (u // tx) % ty
