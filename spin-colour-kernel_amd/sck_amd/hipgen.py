@@ -738,7 +738,7 @@ class _Gen:
         """Class slots per tile (x slots, rows, planes) of the replica-major kernels: a tile's sites and most of
         their neighbours stay within the L2 window of the workgroups one XCD holds in flight."""
         if self.L3 > 1:
-            return (min(self.BX, 8), min(self.L2, 8), min(self.L3, 4))
+            return (min(self.BX, 4), min(self.L2, 4), min(self.L3, 4))
         return (min(self.BX, 32), min(self.L2, 16), 1)
 
     def rm_plan(self):
