@@ -401,3 +401,34 @@ def test_single_spin_site_anisotropy_distribution_is_exact():
     ez, ea = _sphere_average(ax, Ks, h, T)
     for series, exact in ((sz, ez), (sa, ea)):
         assert abs(np.mean(series) - exact) < 3 * np.std(series) / math.sqrt(len(series)) + 2e-3, exact
+
+
+@pytest.mark.gpu
+def test_full_size_nanodisk_batch_matches_oracle():
+    """The C4 batch at its real size (64 nanodisks, radius U[8, 40], thickness U{4..12} from default_rng(2024),
+    16 replicas each, T in [0.3, 1.8]): after each class launch of sweep 0, four (system, replica) pairs, the
+    largest disk among them, match the oracle's decisions from the same state."""
+    from sck_amd.hamiltonian import nanodisk_set
+    systems = nanodisk_set(64, 8.0, 40.0, 4, 12, seed=2024)
+    Bt = BatchSampler(systems, 16, DEV, seed=0, schedule=Schedule())
+    temps = np.linspace(0.3, 1.8, 16).astype(np.float32)
+    Bt.set_temperature([temps] * len(systems))
+    S = Bt.init_spins(seed=0)
+    big = int(np.argmax([s.n_active for s in systems]))
+    picks = [(0, 0), (17, 5), (63, 15), (big, 9)]
+    beta = Bt.beta.cpu().numpy()
+    for c in range(len(Bt.classes)):
+        views = Bt.views(S)
+        x0 = {p: views[p[0]][[p[1]]].cpu().numpy() for p in picks}
+        Bt.state[1] = 0
+        Bt.sweep(S, 1, classes=(c, c + 1))
+        views = Bt.views(S)
+        for i, r in picks:
+            off = int(Bt.rep_off[i])
+            ref, _, mg = O.chain(O.OracleSystem(systems[i]), x0[(i, r)], beta[off:off + 16], Bt.seeds[i], 0, 1,
+                                 reps=[r], class_range=(c, c + 1), margins=True)
+            got = views[i][[r]].cpu().numpy()
+            bad = np.abs(got - ref).max(axis=1) > 1e-5
+            near = np.abs(mg.reshape(bad.shape)) < 1e-5
+            assert not np.any(bad & ~near), f"class {c} system {i} replica {r}"
+            views[i][r] = torch.as_tensor(ref[0], device=DEV)

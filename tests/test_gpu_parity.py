@@ -173,6 +173,35 @@ def test_ising_c1_shape_bit_exact_against_oracle():
         assert np.array_equal(S.cpu().numpy()[reps].astype(np.float32), ref), f"resident={resident}"
 
 
+@pytest.mark.parametrize("workload", ["heisenberg", "xy"])
+def test_full_size_vector_configs_match_oracle(workload):
+    """The replica-major kernels at their real size: Heisenberg bcc 32^3, J1 = 1, J2 = -0.2, Du = 0.1, h = 0.05,
+    B = 8192 (32 temperatures in [0.5, 3]), and XY triangular 2048^2, B = 2048 (16 temperatures in [0.7, 1.1]).
+    For every class of sweep 0, replicas 0, B/2 + 1 and B - 1 are compared with the oracle from the same state:
+    decisions equal except where |beta dE + ln u| < 1e-5; those replicas then continue from the oracle's state."""
+    import bench
+    s, kw, B, T, _ = bench._make(workload)
+    print(f"[gpu] full-size {workload}: B={B}, {s.n_active} sites", flush=True)
+    A = Sampler(s, B, DEV, seed=0)
+    assert A.replica_major
+    A.set_temperature(T)
+    S = bench.device_spins(A, seed=0)
+    reps = [0, B // 2 + 1, B - 1]
+    os_ = O.OracleSystem(s)
+    beta = A.beta.cpu().numpy()
+    for c in range(len(A.classes)):
+        x0 = S[reps].cpu().numpy()
+        A.state[1] = 0
+        A.sweep(S, 1, classes=(c, c + 1))
+        ref, _, mg = O.chain(os_, x0, beta, 0, 0, 1, reps=reps, class_range=(c, c + 1), margins=True, threads=3)
+        got = S[reps].cpu().numpy()
+        bad = np.abs(got - ref).max(axis=1) > 1e-5
+        near = np.abs(mg.reshape(bad.shape)) < 1e-5
+        assert not np.any(bad & ~near), f"class {c}: {int((bad & ~near).sum())} decisions differ"
+        assert int((np.abs(got - x0).max(axis=1) > 0).sum()) > 0            # the class did update sites
+        S[reps] = torch.as_tensor(ref, device=DEV)
+
+
 @pytest.mark.parametrize("acceptance", ["metropolis", "barker"])
 def test_packed_matches_int8_and_oracle(acceptance):
     s = sy.System("Ising sc odd", 1, cr.make("sc"), (5, 5, 5), (True,) * 3, {1: 1.0}, h=(0.2,))
