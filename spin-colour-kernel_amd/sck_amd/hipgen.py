@@ -49,6 +49,7 @@ __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT
 MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP = range(5)
 BLOCK = 256
 XCDS = 8                      # XCDs of an MI355X; consecutive workgroups are dispatched to them round-robin
+RM_WAVES = 7                  # occupancy cap of the replica-major class kernels (waves per SIMD)
 WORD = 32
 GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
 BW_QUEUE_DIV = 4              # independent bitwise mode: the tail queue holds 1/4 of a block's items
@@ -766,7 +767,11 @@ class _Gen:
         tx, ty, tz = self.rm_tile
         ntx, nty = -(-self.BX // tx), -(-self.L2 // ty)
         tsz = tx * ty * tz
-        w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
+        # Heisenberg: at most RM_WAVES waves per SIMD. Left free to target 8, the compiler keeps the cone kernel
+        # at 31 registers by waiting on each neighbour load before issuing the next (9 loads in flight instead
+        # of 37); XY kernels keep all their loads in flight at 8 waves and stay uncapped.
+        cap = f" __attribute__((amdgpu_waves_per_eu(1, {RM_WAVES})))" if self.n == 3 else ""
+        w(f'extern "C" __global__ void __launch_bounds__({BLOCK}){cap} sck_class_{k}(sck_args A) {{')
         w(f"    const int64_t v = (int64_t)blockIdx.x * {chunk} + blockIdx.y + (int64_t)blockIdx.z * gridDim.y;")
         w(f"    if (blockIdx.y + (int64_t)blockIdx.z * gridDim.y >= {chunk} || v >= {G * S}LL) return;")
         if self.rm_SP == 1:
