@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SCK_ABI_VERSION 1
+#define SCK_ABI_VERSION 2
 
 enum sck_status {
     SCK_OK = 0,

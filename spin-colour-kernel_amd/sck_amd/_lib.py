@@ -22,7 +22,7 @@ LIB = PKG / "libsck.so"
 SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
-SCK_ABI_VERSION = 1
+SCK_ABI_VERSION = 2
 
 
 class Args(C.Structure):

@@ -189,3 +189,29 @@ def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
         ok = (j < g.BX) & (y < g.L2) & (z < g.L3)
         np.add.at(seen, (grp[ok], z[ok], y[ok], j[ok]), 1)
     assert seen.min() == 1 and seen.max() == 1
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset the C compiler gives it in include/sck.h, and the
+    structs have the same size (a mismatch would pass the version check and corrupt every launch)."""
+    import subprocess
+    from sck_amd import _lib as L
+    structs = {"sck_args": L.Args, "sck_launch": L.LaunchC, "sck_desc": L.Desc, "sck_mlp_sample": L.MlpSample,
+               "sck_mlp_config": L.MlpConfig, "sck_mlp_adam_config": L.AdamConfig, "sck_width_grid": L.WidthGrid}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sck.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'    printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("    return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = Path(__file__).resolve().parents[1] / "include"
+    subprocess.run(["gcc", "-std=c11", "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
