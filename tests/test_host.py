@@ -120,7 +120,9 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(L.LIB))
     for name in declared:
         assert hasattr(lib, name)
-    assert L.lib().sck_version() == 1
+    hdr = (Path(__file__).resolve().parents[1] / "include" / "sck.h").read_text()
+    version = int(re.search(r"#define SCK_ABI_VERSION (\d+)", hdr).group(1))
+    assert L.lib().sck_version() == version == L.SCK_ABI_VERSION
 
 
 def test_hiprtc_compiles_generated_modules_without_a_device(tmp_path):
