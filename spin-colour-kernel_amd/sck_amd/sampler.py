@@ -112,11 +112,6 @@ class Sampler:
         self.B = int(batch)
         if self.B < 1:
             raise ValueError("batch must be at least 1")
-        self.device = torch.device(device)
-        if self.device.type != "cuda":
-            raise ValueError("the MI355X backend runs on a HIP device (device='cuda'); it has no CPU path")
-        if self.device.index is None:
-            self.device = torch.device("cuda", torch.cuda.current_device())
         n, b, L3, L2, L1 = system.n, system.b, system.L3, system.L2, system.L1
         self.W = -(-self.B // WORD) if self.packed else self.B
         self.Bp = self.W * WORD if self.packed else self.B
@@ -156,6 +151,12 @@ class Sampler:
                 else "reference"
         self.replica_major = layout == "replica"
         self.layout = layout
+        # arguments are validated above, before the device (as the reference does); nothing runs on the host
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("the MI355X backend runs on a HIP device (device='cuda'); it has no CPU path")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta,
                                  replica_major=self.replica_major, rm_tile=tile, rm_chunks=chunks,
                                  rm_sites=sites_per_block)

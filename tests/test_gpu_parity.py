@@ -344,3 +344,10 @@ def test_sync_sweep_matches_oracle():
     x = O.sync(O.OracleSystem(s), S.cpu().numpy(), A.beta.cpu().numpy(), 3, 0, 1)
     A.sync_sweep(S, 1)
     assert np.allclose(S.cpu().numpy(), x, atol=1e-5)
+
+
+def test_bitwise_needs_one_temperature_per_word():
+    A = Sampler(sy.ising("square", 4), 64, DEV, bitwise=True)
+    A.set_temperature(torch.tensor([2.0] * 32 + [3.0] * 32))
+    with pytest.raises(ValueError, match="word"):
+        A.set_temperature(torch.linspace(2.0, 3.0, 64))

@@ -217,3 +217,30 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+@pytest.mark.parametrize("make, msg", [
+    (lambda: sy.heisenberg("sc", 4), "Ising"),
+    (lambda: sy.System("J1-J2", 1, cr.make("sc", (1, 2)), (4, 4, 4), (True,) * 3, {1: 1.0, 2: 0.5}), "uniform"),
+    (lambda: sy.System("DMI", 1, cr.make("square"), (4, 4), (True,) * 2, {1: 1.0}, extra={0: np.eye(1)}), "extra"),
+])
+def test_bitwise_refuses_other_systems_with_the_reference_messages(make, msg):
+    """The reference's refusals (its tests/test_bitwise.py cases): raised before any device is touched."""
+    from sck_amd.sampler import Sampler
+    with pytest.raises(ValueError, match=msg):
+        Sampler(make(), 32, device="cpu", bitwise=True)
+
+
+@pytest.mark.parametrize("kw, msg", [(dict(words_per_program=3, bitwise=True), "power of two"),
+                                     (dict(max_block=48), "power of two"),
+                                     (dict(independent=True), "bitwise=True"),
+                                     (dict(packed=True), None)])
+def test_sampler_argument_refusals(kw, msg):
+    from sck_amd.sampler import Sampler
+    s = sy.ising("square", 4)
+    if msg is None:                              # packed XY is refused, packed Ising is fine (until the device)
+        with pytest.raises(ValueError, match="Ising"):
+            Sampler(sy.xy("square", 4), 32, device="cpu", **kw)
+        return
+    with pytest.raises(ValueError, match=msg):
+        Sampler(s, 32, device="cpu", **kw)
