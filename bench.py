@@ -302,6 +302,7 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", default=None, help="replica-major kernels: class slots per tile, e.g. 8,8,4")
     ap.add_argument("--chunks", type=int, default=None, help="replica-major kernels: 64-replica chunks per wave")
+    ap.add_argument("--resident-threads", type=int, default=None, help="LDS-resident kernel: threads per replica")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference for the HBM counters)")
     ap.add_argument("--schedule", default="auto",
@@ -333,6 +334,8 @@ def main() -> None:
         kw["tile"] = tuple(int(v) for v in a.tile.split(","))
     if a.chunks:
         kw["chunks"] = a.chunks
+    if a.resident_threads:
+        kw["resident_threads"] = a.resident_threads
     A = Sampler(s, B, "cuda:0", seed=0, **kw)
     A.set_temperature(T)
     if a.workload in ("xy", "heisenberg"):

@@ -276,7 +276,7 @@ DEV int wrap_add(int v, int d, int L) {
 
 class _Gen:
     def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False,
-                 rm_tile=None, rm_chunks=None, rm_sites=None):
+                 rm_tile=None, rm_chunks=None, rm_sites=None, resident_threads=None):
         self.s = sys_
         self.mode = mode
         self.B = int(B)
@@ -303,6 +303,11 @@ class _Gen:
         self.rm_R = None if rm_chunks is None else max(1, int(rm_chunks))
         # class sites per workgroup: 1 (the 4 waves share a site, 256 R replicas) or 4 (a site per wave, 64 R)
         self.rm_SP = 1 if rm_sites is None else int(rm_sites)
+        if resident_threads is not None:
+            rt = int(resident_threads)
+            if rt < 64 or rt > 1024 or rt % 64:
+                raise ValueError("resident_threads must be a multiple of 64 in [64, 1024]")
+            self.RESIDENT_THREADS = rt
         if self.rm_SP not in (1, BLOCK // 64):
             raise ValueError(f"replica-major sites per workgroup must be 1 or {BLOCK // 64}")
         esz = 1 if mode == MODE_INT8 else 4
@@ -1806,11 +1811,13 @@ def fits_resident(sys_: System, mode: int) -> bool:
 
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
-             replica_major: bool = False, rm_tile=None, rm_chunks=None, rm_sites=None) -> Module:
+             replica_major: bool = False, rm_tile=None, rm_chunks=None, rm_sites=None,
+             resident_threads=None) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites).generate()
+    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites,
+                resident_threads).generate()

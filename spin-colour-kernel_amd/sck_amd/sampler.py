@@ -89,7 +89,7 @@ class Sampler:
                  words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
                  bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
                  width=None, learned=None, cache_dir=None, resident: bool = True, layout: str = "auto",
-                 tile=None, chunks=None, sites_per_block=None):
+                 tile=None, chunks=None, sites_per_block=None, resident_threads=None):
         self.sys = system
         self.use_resident = bool(resident)
         self.bitwise = bool(bitwise)
@@ -159,7 +159,7 @@ class Sampler:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta,
                                  replica_major=self.replica_major, rm_tile=tile, rm_chunks=chunks,
-                                 rm_sites=sites_per_block)
+                                 rm_sites=sites_per_block, resident_threads=resident_threads)
         self.src = self.module.src
         dev = self.device
         m = system.site_mask()

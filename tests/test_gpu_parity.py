@@ -98,16 +98,19 @@ def test_resident_kernel_equals_class_launches(i):
     """Whole sweeps in LDS (one launch for n sweeps) equal the per-class launches spin for spin."""
     s = SMALL[i]()
     print(f"[gpu] resident == class launches {s.name} {s.box}", flush=True)
-    A, C = Sampler(s, 5, DEV, seed=4), Sampler(s, 5, DEV, seed=4, resident=False)
-    assert A.module.resident is not None
-    for X in (A, C):
-        X.set_temperature(torch.tensor([0.5, 1.0, 2.0, 3.0, 5.0]))
-    S1 = A.init_spins(seed=7)
-    S2 = S1.clone()
-    A.sweep(S1, 3)
+    C = Sampler(s, 5, DEV, seed=4, resident=False)
+    C.set_temperature(torch.tensor([0.5, 1.0, 2.0, 3.0, 5.0]))
+    S0 = C.init_spins(seed=7)
+    S2 = S0.clone()
     C.sweep(S2, 3)
-    assert torch.equal(S1, S2)
-    assert int(A.state[1]) == 3
+    for rt in (None, 64, 1024):                       # the default and both ends of the thread-count range
+        A = Sampler(s, 5, DEV, seed=4, resident_threads=rt)
+        assert A.module.resident is not None
+        A.set_temperature(torch.tensor([0.5, 1.0, 2.0, 3.0, 5.0]))
+        S1 = S0.clone()
+        A.sweep(S1, 3)
+        assert torch.equal(S1, S2), rt
+        assert int(A.state[1]) == 3
 
 
 @pytest.mark.parametrize("make", [
