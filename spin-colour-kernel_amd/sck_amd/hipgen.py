@@ -200,7 +200,8 @@ def ising_table(sys_: System, beta: np.ndarray) -> np.ndarray:
 
 # --------------------------------------------------------------------------------------------- source
 HEADER = r"""
-#include <hip/hip_runtime.h>
+// hipRTC (and hipcc) provide the HIP device runtime without an include; only the compiler's own stdint.h is
+// named, so compilation does not depend on the HIP install's header path (a profiler's environment can hide it)
 #include <stdint.h>
 
 typedef struct sck_args {
