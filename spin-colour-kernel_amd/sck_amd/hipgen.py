@@ -200,8 +200,11 @@ def ising_table(sys_: System, beta: np.ndarray) -> np.ndarray:
 
 # --------------------------------------------------------------------------------------------- source
 HEADER = r"""
-// hipRTC (and hipcc) provide the HIP device runtime without an include; only the compiler's own stdint.h is
-// named, so compilation does not depend on the HIP install's header path (a profiler's environment can hide it)
+// hipRTC provides the HIP device runtime itself, so only an offline hipcc build names the runtime header: a
+// run-time compile then does not depend on the HIP install's header path (a profiler's environment can hide it)
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 typedef struct sck_args {
