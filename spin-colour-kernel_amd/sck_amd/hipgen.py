@@ -312,6 +312,11 @@ class _Gen:
             if rt < 64 or rt > 1024 or rt % 64:
                 raise ValueError("resident_threads must be a multiple of 64 in [64, 1024]")
             self.RESIDENT_THREADS = rt
+        else:
+            # one workgroup per replica: as many threads as a class has sites, 256 to 1024, so that a batch of
+            # a few hundred replicas still fills the SIMDs (C1: 1024 threads, 8 waves per SIMD, 1.5x faster
+            # than 256)
+            self.RESIDENT_THREADS = min(1024, max(256, 64 * -(-(self.ROWS * self.BX) // 64)))
         if self.rm_SP not in (1, BLOCK // 64):
             raise ValueError(f"replica-major sites per workgroup must be 1 or {BLOCK // 64}")
         esz = 1 if mode == MODE_INT8 else 4
@@ -866,7 +871,7 @@ class _Gen:
         w(f"    if (x0 >= {self.P}) return;")
 
     # ---------------------------------------------------------------------------- resident kernel
-    RESIDENT_THREADS = 256
+    RESIDENT_THREADS = 256           # overridden per module (see __init__)
     RESIDENT_LDS_MAX = 60 * 1024
 
     def resident_plan(self):
