@@ -326,6 +326,15 @@ class _Gen:
         self.lines: list[str] = []
 
     # -------------------------------------------------------------------------------------- helpers
+    def word(self, base: str, site: str, T: str = "uint32_t", const: bool = True) -> str:
+        """Site `site` of one packed word column whose base pointer `base` is wave-uniform in the class
+        kernels. The 32-bit byte offset lets the load or store take the scalar-base form (no per-lane 64-bit
+        address arithmetic); it is exact while a column's 4 NS bytes stay below 2^32."""
+        if 4 * self.NS >= 2 ** 32:
+            return f"{base}[{site}]"
+        c = "const " if const else ""
+        return f"(*({c}{T} *)(({c}char *)({base}) + (uint32_t)({site}) * 4u))"
+
     def at(self, ptr: str, comp: int, site: str) -> str:
         """Component `comp` of site `site` of the replica whose offset is `rbase`. Replica-major storage is
         [site][n][B]: a wave-uniform byte address of the site (scalar) plus the lane's 32-bit offset
@@ -398,6 +407,9 @@ class _Gen:
         if d == 0:
             return var, None
         if periodic:
+            if isinstance(L, (int, np.integer)) and L & (L - 1) == 0:
+                # a power-of-two side wraps by masking (two's complement: exact for either sign of d)
+                return f"(({var} + {d}) & {L - 1})", None
             if not isinstance(L, (int, np.integer)) or abs(d) < L:
                 return f"wrap_add({var}, {d}, {L})", None
             return f"((({var} + {d}) % {L} + {L}) % {L})", None
@@ -992,7 +1004,8 @@ class _Gen:
             groups, extras = _groups(s, a)
             w(f"DEV int dec_{a}(const sck_args &A, const int32_t *__restrict__ src, int64_t wbase, int lane, int rep, "
               f"int own, int x, int y, int z, uint32_t key, int *s_out) {{")
-            w("    const int s0 = ((src[wbase + own] >> lane) & 1) ? 1 : -1;")
+            w("    const int32_t *__restrict__ sw = src + wbase;")
+            w(f"    const int s0 = (({self.word('sw', 'own', 'int32_t')} >> lane) & 1) ? 1 : -1;")
             w("    *s_out = s0;")
             gi = 0
             decl_pos = len(self.lines)
@@ -1002,7 +1015,7 @@ class _Gen:
                 for i in members:
                     q, ok = self.nb_index(i)
                     mk = " && A.mask[q]" if self.has_mask else ""
-                    w(f"        {{ const int q = {q}; int t = 0; if (({ok}){mk}) t = ((src[wbase + q] >> lane) & 1) ? 1 : -1; g += t;"
+                    w(f"        {{ const int q = {q}; int t = 0; if (({ok}){mk}) t = (({self.word('sw', 'q', 'int32_t')} >> lane) & 1) ? 1 : -1; g += t;"
                       + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
                 if Jv != 0.0:
                     w(f"        gsum{gi} = g + {len(members)};")
@@ -1147,10 +1160,10 @@ class _Gen:
         w(f"    uint32_t *Sw = S + (int64_t)wd * {nsite};")
         w("    const int lane = threadIdx.x & 63;")
         w(f"    for (int r = 0; r < {R}; ++r) {{")
-        w(f"        const int t = (blockIdx.x * {R} + r) * blockDim.x + threadIdx.x;")
-        w(f"        const int row = t / {self.BX};")
-        w(f"        const int j = t - row * {self.BX};")
-        w(f"        const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
+        w(f"        const uint32_t t = (blockIdx.x * {R} + r) * blockDim.x + threadIdx.x;")
+        w(f"        const int row = (int)(t / {self.BX}u);")
+        w(f"        const int j = (int)(t - (uint32_t)row * {self.BX}u);")
+        w(f"        const int z = (int)((uint32_t)row / {self.L2}u), y = row - z * {self.L2};")
         w(f"        bool live = row < {self.ROWS};")
         w(f"        int x0 = {self.P};")
         for qq in range(self.P - 1, -1, -1):
@@ -1166,7 +1179,7 @@ class _Gen:
         w(f"        const uint32_t ctr = (uint32_t)wd * {nsite}u + (uint32_t)own;")
         w(f"        levels_{a}(st, ctr, key, 0, 2);")
         w("        const bool tail = st.und != 0u;")
-        w("        if (live && !tail) Sw[own] = s0 ^ st.acc;")
+        w(f"        if (live && !tail) {self.word('Sw', 'own', const=False)} = s0 ^ st.acc;")
         w("        const unsigned long long m = __ballot(tail);")
         w("        if (m) {")
         w("            int qb = 0;")
@@ -1181,7 +1194,7 @@ class _Gen:
         w("                } else {")
         w("                    // queue full: finish the item here, with the same level groups in the same order")
         w(f"                    levels_{a}(st, ctr, key, 2, {LEVELS // 4});")
-        w("                    Sw[own] = s0 ^ st.acc;")
+        w(f"                    {self.word('Sw', 'own', const=False)} = s0 ^ st.acc;")
         w("                }")
         w("            }")
         w("        }")
@@ -1196,7 +1209,7 @@ class _Gen:
         for i in range(nd):
             w(f"        st.e[{i}] = has ? q[i][{3 + i}] : 0u; st.P[{i}] = has ? q[i][{3 + nd + i}] : 0u;")
         w(f"        levels_{a}(st, (uint32_t)wd * {nsite}u + (uint32_t)own, key, 2, {LEVELS // 4});")
-        w("        if (has) Sw[own] ^= st.acc;")
+        w(f"        if (has) {self.word('Sw', 'own', const=False)} ^= st.acc;")
         w("    }")
         w("}")
         w()
@@ -1248,8 +1261,8 @@ class _Gen:
             else:
                 w(f"DEV uint32_t flip_{a}(const sck_args &A, const uint32_t *__restrict__ S, int wd, int own, int x, int y, "
                   f"int z, uint32_t key) {{")
-            w(f"    const int64_t wbase = (int64_t)wd * {nsite};")
-            w("    const uint32_t s0 = S[wbase + own];")
+            w(f"    const uint32_t *__restrict__ Sw = S + (int64_t)wd * {nsite};")
+            w(f"    const uint32_t s0 = {self.word('Sw', 'own')};")
             if indep:
                 w("    *s_out = s0;")
             w("    int ze = 0;")
@@ -1258,7 +1271,7 @@ class _Gen:
             for i in self.bw_bonds(a):
                 q, ok = self.nb_index(i)
                 mk = " && A.mask[q]" if self.has_mask else ""
-                w(f"    {{ const int q = {q}; if (({ok}){mk}) {{ ze += 1; uint32_t cy = s0 ^ S[wbase + q];")
+                w(f"    {{ const int q = {q}; if (({ok}){mk}) {{ ze += 1; uint32_t cy = s0 ^ {self.word('Sw', 'q')};")
                 for p in range(NP):
                     if p < NP - 1:
                         w(f"        const uint32_t t{p} = c{p} & cy; c{p} ^= cy; cy = t{p};")
@@ -1321,13 +1334,20 @@ class _Gen:
                 w("        uint32_t r0 = ctr, r1 = 0u, r2 = 0u, r3 = 0u;")
                 w("        philox10(r0, r1, r2, r3, kg, 0u);")
                 w("        const uint32_t rw[4] = {r0, r1, r2, r3};")
+                # level 4g + q reads bit 31 - 4g - q of each P: shift the group's bits to the top once,
+                # then spread bit 31 - q into an all-ones or all-zero mask (uniform P: scalar work)
+                for i in range(nd):
+                    w(f"        const uint32_t Pg{i} = st.P[{i}] << (4 * g);")
                 w("#pragma unroll")
                 w("        for (int q = 0; q < 4; ++q) {")
-                w(f"            const uint32_t sh = {LEVELS - 1} - (4 * g + q);")
-                pb = " | ".join(f"(st.e[{i}] & (0u - ((st.P[{i}] >> sh) & 1u)))" for i in range(nd))
-                w(f"            const uint32_t pb = {pb};")
-                w("            st.acc |= st.und & ~rw[q] & pb;")
-                w("            st.und &= ~(rw[q] ^ pb);")
+                # bitop3 truth tables over (a, b, c) = (0xF0, 0xCC, 0xAA): (a & b) | c = 0xEA,
+                # a & ~b & c = 0x20, a & ~(b ^ c) = 0x90
+                w("            uint32_t pb = st.e[0] & (uint32_t)((int32_t)(Pg0 << q) >> 31);")
+                for i in range(1, nd):
+                    w(f"            pb = __builtin_amdgcn_bitop3_b32(st.e[{i}], (uint32_t)((int32_t)(Pg{i} << q) >> 31), "
+                      f"pb, 0xEA);")
+                w("            st.acc |= __builtin_amdgcn_bitop3_b32(st.und, rw[q], pb, 0x20);")
+                w("            st.und = __builtin_amdgcn_bitop3_b32(st.und, rw[q], pb, 0x90);")
                 w("        }")
                 w("    }")
             w("}")
@@ -1365,7 +1385,8 @@ class _Gen:
                 w("    if (!live) return;")
                 w("    uint32_t *S = (uint32_t *)A.spins;")
                 w("    const uint32_t key = sweep_key(A);")
-                w(f"    S[(int64_t)wd * {nsite} + own] ^= flip_{a}(A, S, wd, own, x, y, z, key);")
+                w(f"    uint32_t *Sw = S + (int64_t)wd * {nsite};")
+                w(f"    {self.word('Sw', 'own', const=False)} ^= flip_{a}(A, S, wd, own, x, y, z, key);")
             w("}")
             w()
         if indep:
