@@ -247,10 +247,15 @@ def nanodisk_main(a) -> None:
     upd = reps * sum(s.n_active for s in systems)
     value = upd * steps / sec
     sch = Bt.schedule
-    # dominant kernel: the class launches (2 per sweep) or, for the chain schedule, one launch per k sweeps
+    # dominant kernel: the class launches (2 per sweep) or, for the chain schedule, one launch per k sweeps;
+    # its own duration from HIP event pairs around each of its launches, over further sweeps after the region
     launches = steps * len(Bt.classes) if sch.kind == "class" else nrep
-    launch_s = sec / launches
     bytes_per_launch = upd * steps / launches * 29.0        # 24 B spin read + write, 1 B mask, 4 B site K
+    kms, kn = 0.0, 0
+    for _ in range(5):
+        m_, n_ = Bt.sweep_timed(S, k)
+        kms, kn = kms + m_, kn + n_
+    launch_s = kms / 1e3 / kn
     achieved = bytes_per_launch / launch_s / 1e9
     # CPU: the oracle's sequential chain on one replica of the median disk
     med = int(np.argsort([s.n_active for s in systems])[len(systems) // 2])
@@ -272,7 +277,8 @@ def nanodisk_main(a) -> None:
                       "parallelism": "single GPU"},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("nanodisks"),
-                        "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6},
+                        "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6,
+                        "launches_timed": kn},
            "autotune": [[lbl, ms] for lbl, ms in Bt.tuning], "wall_s": wall}
     if not a.no_cpu_baseline and sweeps:
         ncores = min(16, reps, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1)
@@ -365,10 +371,16 @@ def main() -> None:
     ms = e0.elapsed_time(e1)
     sec = ms / 1e3
     value = upd_per_sweep * steps / sec
-    # dominant kernel: the class launches (or the resident kernel); the counter bump is negligible
+    # dominant kernel: the class launches (or the resident kernel). Its own duration comes from HIP event
+    # pairs around each of its launches on the sampler's stream, over further sweeps right after the timed
+    # region (the counter bump and launch gaps stay outside the pairs)
     launches = nl if resident else steps * ncls
-    launch_s = sec / launches
     bytes_per_launch = upd_per_sweep * steps / launches * alg_bytes
+    kms, kn = 0.0, 0
+    for _ in range(min(nl, 20) if resident else 1):
+        m_, n_ = A.sweep_timed(S, per if resident else min(steps, 50))
+        kms, kn = kms + m_, kn + n_
+    launch_s = kms / 1e3 / kn
     achieved = bytes_per_launch / launch_s / 1e9
     out = {
         "metric": "site updates/s (%HBM roofline)",
@@ -388,7 +400,8 @@ def main() -> None:
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload),
-                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6},
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6,
+                     "launches_timed": kn},
         "wall_s": wall,
     }
     if not a.no_cpu_baseline:

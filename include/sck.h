@@ -150,6 +150,14 @@ int sck_destroy(sck_handle *h);
  * class_lo/class_hi select launches [lo, hi) of one sweep (0, -1: all); used by per-class checks. */
 int sck_sweep(sck_handle *h, const sck_args *args, int32_t n, int32_t class_lo, int32_t class_hi, void *stream);
 
+/* sck_sweep of n whole sweeps with a HIP event pair on `stream` around every sweep-kernel launch (the class
+ * kernels, or the resident kernel; not the counter bump). Synchronizes the stream, then writes the summed
+ * elapsed milliseconds of those launches to *kernel_ms and their number to *launches. For measuring the
+ * sweep kernel's own duration; no counterpart in the reference (its benchmark times whole sweeps,
+ * SCK/sampler.py:234-240). */
+int sck_sweep_timed(sck_handle *h, const sck_args *args, int32_t n, void *stream, float *kernel_ms,
+                    int32_t *launches);
+
 /* n sequential sweeps (one thread per replica or word visiting the sites in class order), then
  * state[1] += n (SCK/sampler.py:281-293). */
 int sck_reference_sweep(sck_handle *h, const sck_args *args, int32_t n, void *stream);

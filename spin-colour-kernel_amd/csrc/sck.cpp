@@ -179,7 +179,32 @@ int bump(sck_handle *h, const sck_args &a, int64_t by, hipStream_t s) {
     return launch(h, h->bump, b, s);
 }
 
-int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_t hi, hipStream_t s) {
+// Event pairs (start, stop, start, stop, ...) recorded on the launch stream around sweep-kernel launches; the
+// counter bump stays outside them. Destroyed with the timer (a pending event may be destroyed).
+struct launch_timer {
+    std::vector<hipEvent_t> ev;
+    ~launch_timer() {
+        for (hipEvent_t e : ev)
+            if (e) hipEventDestroy(e);
+    }
+};
+
+int timed_launch(sck_handle *h, const sck_launch &l, const sck_args &a, hipStream_t s, launch_timer *t) {
+    if (!t) return launch(h, l, a, s);
+    hipEvent_t e[2] = {nullptr, nullptr};
+    for (hipEvent_t &x : e) {
+        HIP_TRY(hipEventCreate(&x));
+        t->ev.push_back(x);
+    }
+    HIP_TRY(hipEventRecord(e[0], s));
+    int rc = launch(h, l, a, s);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(e[1], s));
+    return SCK_OK;
+}
+
+int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_t hi, hipStream_t s,
+              launch_timer *t = nullptr) {
     int ns = (int)h->sweep.size();
     if (hi < 0 || hi > ns) hi = ns;
     if (lo < 0 || lo > hi) return fail(SCK_E_INVALID, "class range out of bounds");
@@ -188,14 +213,14 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
         // whole sweeps: one launch, each workgroup runs its replica's n sweeps with the lattice in LDS
         a.sweep_offset = 0;
         a.nsweep = n;
-        int rc = launch(h, h->resident, a, s);
+        int rc = timed_launch(h, h->resident, a, s, t);
         if (rc) return rc;
         return bump(h, *args, n, s);
     }
     for (int32_t i = 0; i < n; ++i) {
         a.sweep_offset = i;
         for (int k = lo; k < hi; ++k) {
-            int rc = launch(h, h->sweep[k], a, s);
+            int rc = timed_launch(h, h->sweep[k], a, s, t);
             if (rc) return rc;
         }
     }
@@ -278,6 +303,25 @@ int sck_destroy(sck_handle *h) {
 int sck_sweep(sck_handle *h, const sck_args *args, int32_t n, int32_t class_lo, int32_t class_hi, void *stream) {
     if (!h || !args || n < 0) return fail(SCK_E_INVALID, "invalid sweep arguments");
     return do_sweeps(h, args, n, class_lo, class_hi, (hipStream_t)stream);
+}
+
+int sck_sweep_timed(sck_handle *h, const sck_args *args, int32_t n, void *stream, float *kernel_ms,
+                    int32_t *launches) {
+    if (!h || !args || n <= 0 || !kernel_ms || !launches) return fail(SCK_E_INVALID, "invalid timed-sweep arguments");
+    hipStream_t s = (hipStream_t)stream;
+    launch_timer t;
+    int rc = do_sweeps(h, args, n, 0, -1, s, &t);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    float total = 0.0f;
+    for (size_t i = 0; i + 1 < t.ev.size(); i += 2) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));
+        total += ms;
+    }
+    *kernel_ms = total;
+    *launches = (int32_t)(t.ev.size() / 2);
+    return SCK_OK;
 }
 
 int sck_reference_sweep(sck_handle *h, const sck_args *args, int32_t n, void *stream) {

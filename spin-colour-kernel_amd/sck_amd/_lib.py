@@ -53,7 +53,7 @@ class Desc(C.Structure):
 
 #: every symbol include/sck.h declares
 EXPORTS = ["sck_version", "sck_last_error", "sck_create", "sck_precompile", "sck_destroy", "sck_sweep",
-           "sck_reference_sweep", "sck_sync_sweep", "sck_observe", "sck_graph_capture", "sck_graph_replay",
+           "sck_sweep_timed", "sck_reference_sweep", "sck_sync_sweep", "sck_observe", "sck_graph_capture", "sck_graph_replay",
            "sck_graph_destroy", "sck_time_replays", "sck_mlp_nparam", "sck_mlp_workspace_bytes",
            "sck_mlp_train_step", "sck_mlp_table", "sck_sample_sites"]
 
@@ -111,6 +111,8 @@ def lib():
         L.sck_precompile.argtypes = [C.c_char_p, C.c_char_p]
         L.sck_destroy.argtypes = [C.c_void_p]
         L.sck_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        L.sck_sweep_timed.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_void_p, C.POINTER(C.c_float),
+                                      C.POINTER(C.c_int32)]
         L.sck_reference_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_void_p]
         L.sck_sync_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int64, C.c_void_p]
         L.sck_observe.argtypes = [C.c_void_p, C.POINTER(Args), C.c_void_p]

@@ -453,6 +453,20 @@ class BatchSampler:
                                       C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)), "batch sweep")
         return S
 
+    def sweep_timed(self, S: torch.Tensor, n: int) -> tuple[float, int]:
+        """n sweeps as sweep(), with HIP events around every sweep-kernel launch; returns the summed kernel
+        milliseconds and the number of launches (the counter bump is not counted). Synchronizes."""
+        self._check(S)
+        if n <= 0:
+            raise ValueError("n must be positive")
+        a = self._args(S)
+        ms, nl = C.c_float(), C.c_int32()
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_sweep_timed(self._handle, C.byref(a), int(n),
+                                            C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream),
+                                            C.byref(ms), C.byref(nl)), "batch timed sweep")
+        return float(ms.value), int(nl.value)
+
     def reference_sweep(self, S: torch.Tensor, n: int = 1) -> torch.Tensor:
         """Sequential chain of every (system, replica): classes in order, sites in raster order."""
         self._check(S)

@@ -919,7 +919,11 @@ class _Gen:
             w(f"    __shared__ float tbl[{total}];")
             w(f"    __shared__ int q_own[{chunk}];")
             w(f"    __shared__ float q_p[{chunk}];")
-            w("    __shared__ int q_n;")
+            # two queue counters: a pass fills q_n[par] while the other is cleared for the next pass, so no
+            # barrier is spent on the reset
+            w("    __shared__ int q_n[2];")
+            w("    if (threadIdx.x < 2) q_n[threadIdx.x] = 0;")
+            w("    int par = 0;")
         w("    const int rep = blockIdx.x;")
         w("    if (rep >= A.nrep) return;")
         w(f"    {T} *g = ({T} *)A.spins + (int64_t)rep * {NSn};")
@@ -937,8 +941,6 @@ class _Gen:
             w(f"        // class {k}: colour {c}, sublattice {a}")
             if self.n == 1:
                 w(f"        for (int base = 0; base < {slots}; base += {chunk}) {{")
-                w("            if (threadIdx.x == 0) q_n = 0;")
-                w("            __syncthreads();")
                 w(f"            for (int t = base + threadIdx.x; t < base + {chunk}; t += {RT}) {{")
             else:
                 w(f"        for (int t = threadIdx.x; t < {-(-slots // RT) * RT}; t += {RT}) {{")
@@ -962,7 +964,7 @@ class _Gen:
                 w("                const unsigned long long m = __ballot(need);")
                 w("                if (m) {")
                 w("                    int qb = 0;")
-                w("                    if (lane == 0) qb = atomicAdd(&q_n, __popcll(m));")
+                w("                    if (lane == 0) qb = atomicAdd(&q_n[par], __popcll(m));")
                 w("                    qb = __shfl(qb, 0, 64);")
                 w("                    if (need) {")
                 w("                        const int pos = qb + __popcll(m & ((1ull << lane) - 1ull));")
@@ -972,7 +974,8 @@ class _Gen:
                 w("                }")
                 w("            }")
                 w("            __syncthreads();")
-                w("            const int nq = q_n;")
+                w("            const int nq = q_n[par];")
+                w("            if (threadIdx.x == 0) q_n[par ^ 1] = 0;")
                 w(f"            for (int e = threadIdx.x; e < nq; e += {RT}) {{")
                 w("                const int own = q_own[e];")
                 w("                const float p = q_p[e];")
@@ -980,6 +983,7 @@ class _Gen:
                 w(f"                if ({self.accept_expr('p')}) lat[own] = ({T})(-(int)lat[own]);")
                 w("            }")
                 w("            __syncthreads();")
+                w("            par ^= 1;")
                 w("        }")
             else:
                 w(f"                if (valid) upd_{a}(Ai, lat, lat, 0, rep, own, x, y, z, key);")

@@ -387,6 +387,21 @@ class Sampler:
                                           C.c_void_p(_stream_ptr(self.device))), "sweep")
         return self._run(S, run)
 
+    def sweep_timed(self, S: torch.Tensor, n: int) -> tuple[float, int]:
+        """n whole sweeps in place, as sweep(), with HIP events around every sweep-kernel launch; returns the
+        summed kernel milliseconds and the number of launches (the counter bump is not counted). Synchronizes."""
+        if n <= 0:
+            raise ValueError("n must be positive")
+        self._check_spins(S)
+        if not self._is_native(S):
+            raise ValueError("timed sweeps need spins in the sampler's storage order (as init_spins returns them)")
+        a = self._args(S)
+        ms, nl = C.c_float(), C.c_int32()
+        with torch.cuda.device(self.device):
+            L.check(L.lib().sck_sweep_timed(self._handle, C.byref(a), int(n), C.c_void_p(_stream_ptr(self.device)),
+                                            C.byref(ms), C.byref(nl)), "timed sweep")
+        return float(ms.value), int(nl.value)
+
     def graph(self, S: torch.Tensor, k: int, observe: bool = False):
         """Capture k sweeps (and, with observe=True, one observation after them) in a HIP graph; returns its
         replay (SCK/sampler.py:242-269). The replay keeps the sampler and S alive."""

@@ -339,6 +339,28 @@ def test_graph_replay_equals_eager_sweeps_and_reads_beta_by_address():
     assert int(A.state[1]) == 12
 
 
+@pytest.mark.parametrize("resident", [False, True])
+def test_timed_sweeps_equal_plain_sweeps(resident):
+    """sck_sweep_timed runs the same sweeps (spins and counter) and counts one event pair per sweep-kernel
+    launch: one per class per sweep, or one resident launch for all n sweeps."""
+    s = sy.ising("square", 16)
+    print(f"[gpu] timed sweeps resident={resident}", flush=True)
+    A, Bs = (Sampler(s, 64, DEV, seed=3, packed=False, resident=resident) for _ in range(2))
+    assert (A.module.resident is not None and A.use_resident) == resident
+    for X in (A, Bs):
+        X.set_temperature(torch.linspace(1.0, 4.0, 64))
+    S1 = A.init_spins(seed=2)
+    S2 = S1.clone()
+    ms, nl = A.sweep_timed(S1, 4)
+    Bs.sweep(S2, 4)
+    torch.cuda.synchronize()
+    assert torch.equal(S1, S2) and int(A.state[1]) == 4
+    assert nl == (1 if resident else 4 * len(A.classes))
+    assert ms > 0.0
+    with pytest.raises(ValueError):
+        A.sweep_timed(S1, 0)
+
+
 def test_sync_sweep_matches_oracle():
     s = sy.xy("square", 8)
     A = Sampler(s, 4, DEV, seed=3)
