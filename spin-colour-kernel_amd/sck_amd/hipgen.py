@@ -234,6 +234,33 @@ DEV void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32
     }
 }
 
+// philox10 for the counter (c0, 0, 0, 0) and key (k0, 0), the form every sweep draw uses. Round 1 has one
+// product (the other multiplies c2 = 0) and leaves c0 = k0, c1 = 0, so round 2's second product depends on the
+// key alone and is uniform across a wave when the key is; rounds 3-10 are the general ones.
+DEV void philox10_z(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0) {
+    const uint64_t q = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    const uint64_t p0 = (uint64_t)0xCD9E8D57u * (uint64_t)(uint32_t)(q >> 32);
+    const uint64_t pk = (uint64_t)0xD2511F53u * (uint64_t)k0;
+    c0 = (uint32_t)(p0 >> 32) ^ (k0 + 0x9E3779B9u);
+    c1 = (uint32_t)p0;
+    c2 = (uint32_t)q ^ ((uint32_t)(pk >> 32) ^ 0xBB67AE85u);
+    c3 = (uint32_t)pk;
+    uint32_t k = k0 + 2u * 0x9E3779B9u, l = 2u * 0xBB67AE85u;
+#pragma unroll
+    for (int r = 2; r < 10; ++r) {
+        const uint64_t a = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint64_t b = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), c1, k, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(b >> 32), c3, l, 0x96);
+        c1 = (uint32_t)a;
+        c3 = (uint32_t)b;
+        c0 = n0;
+        c2 = n2;
+        k += 0x9E3779B9u;
+        l += 0xBB67AE85u;
+    }
+}
+
 // uniform in [0, 1): fp32(x < 0 ? ~x : x) * fp32(4.6566127342e-10), x the output read as int32
 DEV float u01(uint32_t x) {
     int32_t v = (int32_t)x;
@@ -614,7 +641,10 @@ class _Gen:
         else:
             ns = f"{self.NS}u" if not self.rt else f"(uint32_t){self.NS}"
             w(f"{ind}uint32_t c0 = (uint32_t)rep * {ns} + (uint32_t)own, c1 = 0u, c2 = 0u, c3 = 0u;")
-        w(f"{ind}philox10(c0, c1, c2, c3, key, 0u);")
+        if self.wide:
+            w(f"{ind}philox10(c0, c1, c2, c3, key, 0u);")
+        else:
+            w(f"{ind}philox10_z(c0, c1, c2, c3, key);")
         w(f"{ind}const float u0 = u01(c0);")
         if self.n > 1:
             w(f"{ind}const float u1 = u01(c1);")
@@ -1165,10 +1195,15 @@ class _Gen:
         w("    const int lane = threadIdx.x & 63;")
         w(f"    for (int r = 0; r < {R}; ++r) {{")
         w(f"        const uint32_t t = (blockIdx.x * {R} + r) * blockDim.x + threadIdx.x;")
-        w(f"        const int row = (int)(t / {self.BX}u);")
+        if self.BX % 64 == 0:
+            # a wave's 64 consecutive slots lie in one row: row, y, z and the colour offset are wave-uniform
+            w(f"        const int row = (int)(__builtin_amdgcn_readfirstlane(t) / {self.BX}u);")
+        else:
+            w(f"        const int row = (int)(t / {self.BX}u);")
         w(f"        const int j = (int)(t - (uint32_t)row * {self.BX}u);")
         w(f"        const int z = (int)((uint32_t)row / {self.L2}u), y = row - z * {self.L2};")
-        w(f"        bool live = row < {self.ROWS};")
+        w(f"        const bool rowok = row < {self.ROWS};")
+        w("        bool live = rowok;")
         w(f"        int x0 = {self.P};")
         for qq in range(self.P - 1, -1, -1):
             w(f"        if ({self.colour_expr(str(qq), a)} == {c}) x0 = {qq};")
@@ -1178,7 +1213,9 @@ class _Gen:
         if self.has_mask:
             w("        live = live && A.mask[own];")
         w("        uint32_t s0;")
-        w(f"        BW{a} st = setup_{a}(A, S, wd, own, live ? x : 0, live ? y : 0, live ? z : 0, &s0);")
+        # y and z stay in range whenever the row is (a dead lane's x is zeroed): with a wave-uniform row the
+        # coordinates and their neighbour offsets stay scalar
+        w(f"        BW{a} st = setup_{a}(A, S, wd, own, live ? x : 0, rowok ? y : 0, rowok ? z : 0, &s0);")
         w("        if (!live) { st.acc = 0u; st.und = 0u; }")
         w(f"        const uint32_t ctr = (uint32_t)wd * {nsite}u + (uint32_t)own;")
         w(f"        levels_{a}(st, ctr, key, 0, 2);")
@@ -1289,7 +1326,7 @@ class _Gen:
             if not indep:
                 w(f"    const float *tb = (const float *)A.table + (int64_t)wd * {NT} + ze * {Z1};")
                 w(f"    uint32_t c0r = (uint32_t)(wd >> 2) * {nsite}u + (uint32_t)own, c1r = 0u, c2r = 0u, c3r = 0u;")
-                w("    philox10(c0r, c1r, c2r, c3r, key, 0u);")
+                w("    philox10_z(c0r, c1r, c2r, c3r, key);")
                 w("    const int qsel = wd & 3;")
                 w("    const uint32_t raw = qsel == 0 ? c0r : (qsel == 1 ? c1r : (qsel == 2 ? c2r : c3r));")
                 w("    const float u = u01(raw);")
@@ -1334,9 +1371,12 @@ class _Gen:
             if nd:
                 w("    for (int g = g0; g < g1; ++g) {")
                 w("        if (__ballot(st.und != 0u) == 0ull) break;")
-                w(f"        const uint32_t kg = (uint32_t)(((int64_t)key + (int64_t)(g + 1) * {GROUP_KEY_STEP}LL) % 2147483629LL);")
+                # (key + (g + 1) * step) mod 2147483629 with key already reduced and (g + 1) * step < 2^20: one
+                # conditional subtract reduces it (scalar work per level group)
+                w(f"        const uint32_t kx = key + (uint32_t)(g + 1) * {GROUP_KEY_STEP}u;")
+                w("        const uint32_t kg = kx >= 2147483629u ? kx - 2147483629u : kx;")
                 w("        uint32_t r0 = ctr, r1 = 0u, r2 = 0u, r3 = 0u;")
-                w("        philox10(r0, r1, r2, r3, kg, 0u);")
+                w("        philox10_z(r0, r1, r2, r3, kg);")
                 w("        const uint32_t rw[4] = {r0, r1, r2, r3};")
                 # level 4g + q reads bit 31 - 4g - q of each P: shift the group's bits to the top once,
                 # then spread bit 31 - q into an all-ones or all-zero mask (uniform P: scalar work)
