@@ -472,6 +472,13 @@ class Sampler:
         B, b, n = self.B, self.sys.b, self.sys.n
         return self._obs[:B].clone(), self._obs[B:].reshape(B, b, n).clone()
 
+    def observed(self):
+        """Views (E (B,), M (B, b, n)) of the float64 observation buffers: the last observe(), or the
+        observation a graph captured with observe=True wrote on its last replay (read after the replay's
+        stream has reached it). They are overwritten by the next observation."""
+        B, b, n = self.B, self.sys.b, self.sys.n
+        return self._obs[:B], self._obs[B:].reshape(B, b, n)
+
     def energy(self, S: torch.Tensor) -> torch.Tensor:
         """Total energy per replica, float64 (SCK/sampler.py:326-341)."""
         if self.packed and S.dtype != torch.int32:

@@ -125,3 +125,64 @@ def test_structure_factor_of_the_120_degree_state():
     ferro = torch.zeros_like(S)
     ferro[:, 0] = 1.0
     assert abs(structure_factor(ferro, s, [0.0, 0.0]).numpy()[0] - L * L) < 1e-9
+
+
+# ------------------------------------------------------------------------------ multi-histogram reweighting
+def _exact_samples(s, temps, n, seed):
+    """Independent canonical energy samples of an Ising system at each temperature, from its exact density
+    of states (no Markov chain), with the exact levels and degeneracies."""
+    rng = np.random.default_rng(seed)
+    ref = exact_energy_distribution(s, temps[0])
+    lv, g = ref["levels"], ref["degeneracy"].astype(float)
+    out = []
+    for T in temps:
+        w = g * np.exp(-(lv - lv.min()) / T)
+        out.append(rng.choice(lv, size=n, p=w / w.sum()))
+    return np.array(out), lv, g
+
+
+def test_multi_histogram_reproduces_exact_canonical_averages():
+    """Samples at T = 2.0, 2.3, 2.6, 3.0 of the 4x4 Ising model reweighted to temperatures between and
+    beyond them agree with <E> and <E^2> from exact enumeration; the free energies agree with exact ln Z."""
+    from sck_amd.reweight import multi_histogram
+    s = sy.ising("square", 4)
+    temps = np.array([2.0, 2.3, 2.6, 3.0])
+    E, lv, g = _exact_samples(s, temps, 40000, seed=3)
+    grid = np.array([2.0, 2.15, 2.45, 2.8, 3.0])
+    (mE, mE2), lnZ = multi_histogram(E, [E, E * E], temps, grid, return_lnZ=True)
+    for T, a, a2 in zip(grid, mE, mE2):
+        w = g * np.exp(-(lv - lv.min()) / T)
+        p = w / w.sum()
+        ex, ex2 = float(p @ lv), float(p @ lv ** 2)
+        se = math.sqrt(ex2 - ex * ex) / math.sqrt(40000)
+        assert abs(a - ex) < 5 * se, (T, a, ex, se)
+        assert abs(a2 - ex2) < 5 * se * 2 * abs(ex) + 1e-9, (T, a2, ex2)
+    e0 = E.mean()
+    lz = np.array([np.log(np.sum(g * np.exp(-(lv - e0) / T))) for T in temps])
+    assert np.allclose(lnZ, lz - lz[0], atol=0.02), (lnZ, lz - lz[0])
+
+
+def test_multi_histogram_identities():
+    """One temperature evaluated at itself gives the sample mean; a weight of 2 equals a duplicated sample;
+    a zero weight drops a sample."""
+    from sck_amd.reweight import multi_histogram
+    rng = np.random.default_rng(1)
+    E = rng.normal(-20.0, 3.0, (1, 500))
+    O = rng.normal(0.0, 1.0, (1, 500))
+    (m,) = multi_histogram(E, [O], [2.5], [2.5])
+    assert abs(m[0] - O.mean()) < 1e-12
+    E2 = rng.normal(-20.0, 3.0, (2, 300))
+    O2 = E2 ** 2
+    w = np.ones_like(E2)
+    w[0, 5] = 2.0
+    w[1, 7] = 0.0
+    a = multi_histogram(E2, [O2], [2.0, 2.4], [2.2], weights=w)[0]
+    Ed = [np.concatenate([E2[0], E2[0, 5:6]]), np.delete(E2[1], 7)]
+    n = max(len(r) for r in Ed)
+    pad = lambda r: np.concatenate([r, np.zeros(n - len(r))])                   # noqa: E731
+    wd = np.array([pad(np.ones(len(r))) for r in Ed])
+    Epad = np.array([pad(r) for r in Ed])
+    b = multi_histogram(Epad, [Epad ** 2], [2.0, 2.4], [2.2], weights=wd)[0]
+    assert abs(a[0] - b[0]) < 1e-9 * abs(b[0])
+    with pytest.raises(ValueError):
+        multi_histogram(E2, [O2[:, :3]], [2.0, 2.4], [2.2])
