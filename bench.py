@@ -374,7 +374,7 @@ def main() -> None:
     # dominant kernel: the class launches (or the resident kernel). Its own duration comes from HIP event
     # pairs around each of its launches on the sampler's stream, over further sweeps right after the timed
     # region (the counter bump and launch gaps stay outside the pairs)
-    launches = nl if resident else steps * ncls
+    launches = nl if resident else steps * len(A.module.sweep)     # class launches (per replica block)
     bytes_per_launch = upd_per_sweep * steps / launches * alg_bytes
     kms, kn = 0.0, 0
     for _ in range(min(nl, 20) if resident else 1):
@@ -397,6 +397,7 @@ def main() -> None:
         "data": "synthetic (random initial spins from the seeded torch CPU generator)",
         "config": {"workload": WORKLOADS[a.workload], "batch": B, "sites": s.n_active, "classes": ncls,
                    "kernel": "sck_resident" if resident else "sck_class_*", "layout": A.layout,
+                   "replica_blocks": A.module.rep_blocks,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload),

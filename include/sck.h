@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SCK_ABI_VERSION 2
+#define SCK_ABI_VERSION 3
 
 enum sck_status {
     SCK_OK = 0,
@@ -61,7 +61,7 @@ typedef struct sck_args {
     const void *aux;          /* batched systems: the per-system descriptor table                  */
     const int32_t *tiles;     /* batched systems: the launch's work list (set from sck_launch)     */
     int32_t ntile;            /* its length                                                        */
-    int32_t pad0;
+    int32_t rep_base;         /* first replica of the launch's replica block (set from sck_launch) */
 } sck_args;
 
 /* One kernel launch of a plan. */
@@ -72,7 +72,7 @@ typedef struct sck_launch {
     uint32_t shmem;
     const int32_t *tiles;     /* NULL, or the work list handed to the kernel as args.tiles / args.ntile */
     int32_t ntile;
-    int32_t pad0;
+    int32_t rep_base;         /* replica-major class launches: first replica of the block it covers, else 0 */
 } sck_launch;
 
 /* Launch plans of a system, computed by the host-side generator. */
@@ -91,6 +91,11 @@ typedef struct sck_desc {
     const sck_launch *bump;                      /* state[1] += sweep_offset                          */
     const sck_launch *resident;                  /* NULL, or whole sweeps with each replica's lattice in LDS:
                                                     one launch performs args.nsweep sweeps                */
+    int32_t rep_blocks;       /* >= 1: the sweep list is rep_blocks groups of n_sweep / rep_blocks class
+                                 launches, one group per replica block. sck_sweep runs each block through
+                                 all n sweeps before the next (replicas are independent chains, so the chain
+                                 of every replica is unchanged) and a block's lattice stays cache-resident */
+    int32_t pad1;
 } sck_desc;
 
 /* ---------------------------------------------------------------- learned proposal width (no reference

@@ -150,6 +150,7 @@ struct sck_handle {
     sck_launch resident{};
     int sample_kernel = -1;
     int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
+    int rep_blocks = 1;         // the sweep list holds rep_blocks groups of class launches, one per replica block
 };
 
 struct sck_graph {
@@ -163,6 +164,7 @@ namespace {
 int launch(sck_handle *h, const sck_launch &l, const sck_args &a, hipStream_t s) {
     if (l.kernel < 0 || l.kernel >= (int)h->fn.size()) return fail(SCK_E_INVALID, "launch plan names an unknown kernel");
     sck_args copy = a;
+    copy.rep_base = l.rep_base;
     if (l.tiles) {
         copy.tiles = l.tiles;
         copy.ntile = l.ntile;
@@ -205,7 +207,8 @@ int timed_launch(sck_handle *h, const sck_launch &l, const sck_args &a, hipStrea
 
 int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_t hi, hipStream_t s,
               launch_timer *t = nullptr) {
-    int ns = (int)h->sweep.size();
+    // lo, hi: class range; the sweep list is rep_blocks groups of ns class launches
+    const int nb = h->rep_blocks, ns = (int)h->sweep.size() / nb;
     if (hi < 0 || hi > ns) hi = ns;
     if (lo < 0 || lo > hi) return fail(SCK_E_INVALID, "class range out of bounds");
     sck_args a = *args;
@@ -217,11 +220,14 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
         if (rc) return rc;
         return bump(h, *args, n, s);
     }
-    for (int32_t i = 0; i < n; ++i) {
-        a.sweep_offset = i;
-        for (int k = lo; k < hi; ++k) {
-            int rc = timed_launch(h, h->sweep[k], a, s, t);
-            if (rc) return rc;
+    // each replica block runs all n sweeps before the next one starts
+    for (int rb = 0; rb < nb; ++rb) {
+        for (int32_t i = 0; i < n; ++i) {
+            a.sweep_offset = i;
+            for (int k = lo; k < hi; ++k) {
+                int rc = timed_launch(h, h->sweep[(size_t)rb * ns + k], a, s, t);
+                if (rc) return rc;
+            }
         }
     }
     return n > 0 ? bump(h, *args, n, s) : SCK_OK;
@@ -253,6 +259,8 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
     if (!d || !hip_src || !out) return fail(SCK_E_INVALID, "null argument");
     if (d->abi_version != SCK_ABI_VERSION) return fail(SCK_E_INVALID, "descriptor ABI version mismatch");
     if (d->n_kernels <= 0 || !d->kernel_names) return fail(SCK_E_INVALID, "no kernels");
+    if (d->rep_blocks < 1 || d->n_sweep % d->rep_blocks != 0)
+        return fail(SCK_E_INVALID, "rep_blocks must be >= 1 and divide the sweep launches");
     std::string code;
     int rc = code_object(hip_src, cache_dir, code);
     if (rc) return rc;
@@ -289,6 +297,7 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
     h->batch = d->batch;
     h->words = d->words;
     h->obs_chunks = d->obs_chunks;
+    h->rep_blocks = d->rep_blocks;
     *out = h;
     return SCK_OK;
 }

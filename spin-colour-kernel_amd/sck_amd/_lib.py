@@ -22,7 +22,7 @@ LIB = PKG / "libsck.so"
 SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
-SCK_ABI_VERSION = 2
+SCK_ABI_VERSION = 3
 
 
 class Args(C.Structure):
@@ -31,12 +31,12 @@ class Args(C.Structure):
                 ("site_ku", C.c_void_p), ("site_axis", C.c_void_p), ("obs", C.c_void_p),
                 ("obs_scratch", C.c_void_p), ("sweep_offset", C.c_int64), ("nrep", C.c_int32),
                 ("nsweep", C.c_int32), ("aux", C.c_void_p), ("tiles", C.c_void_p), ("ntile", C.c_int32),
-                ("pad0", C.c_int32)]
+                ("rep_base", C.c_int32)]
 
 
 class LaunchC(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("grid", C.c_uint32 * 3), ("block", C.c_uint32 * 3), ("shmem", C.c_uint32),
-                ("tiles", C.c_void_p), ("ntile", C.c_int32), ("pad0", C.c_int32)]
+                ("tiles", C.c_void_p), ("ntile", C.c_int32), ("rep_base", C.c_int32)]
 
 
 class Desc(C.Structure):
@@ -48,7 +48,8 @@ class Desc(C.Structure):
                 ("ref", C.POINTER(LaunchC)), ("n_ref", C.c_int32),
                 ("sync", C.POINTER(LaunchC)), ("n_sync", C.c_int32),
                 ("obs", C.POINTER(LaunchC)), ("n_obs", C.c_int32),
-                ("bump", C.POINTER(LaunchC)), ("resident", C.POINTER(LaunchC))]
+                ("bump", C.POINTER(LaunchC)), ("resident", C.POINTER(LaunchC)), ("rep_blocks", C.c_int32),
+                ("pad1", C.c_int32)]
 
 
 #: every symbol include/sck.h declares
@@ -152,6 +153,7 @@ def launches(ls) -> tuple:
         arr[i].shmem = int(l.shmem)
         arr[i].tiles = getattr(l, "tiles_ptr", None)
         arr[i].ntile = int(getattr(l, "ntile", 0))
+        arr[i].rep_base = int(getattr(l, "rep_base", 0))
     return arr, len(ls)
 
 

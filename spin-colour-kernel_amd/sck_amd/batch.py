@@ -324,6 +324,7 @@ class BatchSampler:
         d.n, d.b = self.n, self.b
         d.L1 = d.L2 = d.L3 = 0
         d.batch, d.words, d.obs_chunks = self.R, self.R, 1
+        d.rep_blocks = 1
         d.kernel_names = C.cast(names, C.POINTER(C.c_char_p))
         d.n_kernels = len(mod.kernels)
         for field_, plan in (("sweep", sweep), ("ref", ref), ("sync", []), ("obs", obs)):

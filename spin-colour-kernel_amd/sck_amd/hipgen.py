@@ -36,7 +36,7 @@ from __future__ import annotations
 
 import hashlib
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -54,6 +54,7 @@ WORD = 32
 GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
 BW_QUEUE_DIV = 4              # independent bitwise mode: the tail queue holds 1/4 of a block's items
 LEVELS = 32
+RM_BLOCK_BYTES = 224 << 20    # replica-major: largest lattice share run through whole sweeps at a time (cache-resident)
 
 
 @dataclass
@@ -64,6 +65,7 @@ class Launch:
     shmem: int = 0
     tiles_ptr: int = 0        # device work list handed to the kernel (batched systems), 0 for none
     ntile: int = 0
+    rep_base: int = 0         # replica-major class launches: first replica of the block the launch covers
 
 
 @dataclass
@@ -81,6 +83,7 @@ class Module:
     ising_layout: list = field(default_factory=list)
     key: str = ""
     resident: Launch | None = None     # whole sweeps of one replica per workgroup with the lattice in LDS
+    rep_blocks: int = 1                # the sweep list is this many groups of class launches, one per replica block
 
 
 def _hexf(v: float) -> str:
@@ -211,7 +214,7 @@ typedef struct sck_args {
     void *spins; void *dst; const int8_t *mask; const float *k1; const float *beta; const void *table;
     const int64_t *state; const float *width; const float *site_ku; const float *site_axis;
     double *obs; double *obs_scratch; int64_t sweep_offset; int32_t nrep; int32_t nsweep;
-    const void *aux; const int32_t *tiles; int32_t ntile; int32_t pad0;
+    const void *aux; const int32_t *tiles; int32_t ntile; int32_t rep_base;
 } sck_args;
 
 #define DEV static __device__ __forceinline__
@@ -307,7 +310,7 @@ DEV int wrap_add(int v, int d, int L) {
 
 class _Gen:
     def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False,
-                 rm_tile=None, rm_chunks=None, rm_sites=None, resident_threads=None):
+                 rm_tile=None, rm_chunks=None, rm_sites=None, resident_threads=None, rm_block=None):
         self.s = sys_
         self.mode = mode
         self.B = int(B)
@@ -347,6 +350,20 @@ class _Gen:
         if self.rm_SP not in (1, BLOCK // 64):
             raise ValueError(f"replica-major sites per workgroup must be 1 or {BLOCK // 64}")
         esz = 1 if mode == MODE_INT8 else 4
+        self.rm_block = self.B
+        if self.replica_major:
+            if rm_block is not None and int(rm_block) not in (0, self.B):
+                rb = int(rm_block)
+                if rb < 64 or rb % 64 or self.B % rb:
+                    raise ValueError("a replica block must be a multiple of 64 that divides the batch")
+                self.rm_block = rb
+            elif rm_block is None:
+                # a lattice larger than the cache runs in replica blocks whose share stays cache-resident
+                per = self.n * self.NS * esz
+                if per * self.B > RM_BLOCK_BYTES:
+                    fits = [c for c in range(64, self.B, 64) if self.B % c == 0 and c * per <= RM_BLOCK_BYTES]
+                    if fits:
+                        self.rm_block = max(fits)
         # one replica's spins addressed by a 32-bit byte offset from its base pointer
         self.u32 = mode in (MODE_INT8, MODE_F32) and self.n * self.NS * esz < 2 ** 31
         self.rt = False               # box dimensions are compile-time constants (runtime: batch kernels)
@@ -804,7 +821,7 @@ class _Gen:
         tx, ty, tz = self.rm_tile
         ntile = -(-self.BX // tx) * -(-self.L2 // ty) * -(-self.L3 // tz)
         S = -(-(ntile * tx * ty * tz) // self.rm_SP)
-        G = -(-self.B // (BLOCK * self.rm_chunks() // self.rm_SP))
+        G = -(-self.rm_block // (BLOCK * self.rm_chunks() // self.rm_SP))
         return G, S, -(-(G * S) // XCDS)
 
     def emit_rm_class(self, k: int, c: int, a: int, T: str) -> None:
@@ -832,13 +849,13 @@ class _Gen:
         w(f"    if (blockIdx.y + (int64_t)blockIdx.z * gridDim.y >= {chunk} || v >= {G * S}LL) return;")
         if self.rm_SP == 1:
             w(f"    const int grp = (int)(v / {S}), t = (int)(v - (int64_t)grp * {S});")
-            w(f"    const int rep0 = grp * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
+            w(f"    const int rep0 = A.rep_base + grp * {BLOCK * R} + (threadIdx.x >> 6) * {64 * R} + (threadIdx.x & 63);")
         else:
             # a site per wave: the wave index made scalar so the site arithmetic stays in scalar registers
             w("    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);")
             w(f"    const int grp = (int)(v / {S});")
             w(f"    const int t = (int)(v - (int64_t)grp * {S}) * {self.rm_SP} + wave;")
-            w(f"    const int rep0 = grp * {64 * R} + (threadIdx.x & 63);")
+            w(f"    const int rep0 = A.rep_base + grp * {64 * R} + (threadIdx.x & 63);")
         w(f"    const int tt = t / {tsz}, u = t - tt * {tsz};")
         w(f"    const int tj = tt % {ntx}, tr = tt / {ntx};")
         w(f"    const int uy = (u / {tx}) % {ty}, uz = u / {tx * ty};")
@@ -857,7 +874,8 @@ class _Gen:
         # one wave, one site, R chunks of 64 replicas: the site's scalar address work is done once
         w(f"    for (int i = 0; i < {R}; ++i) {{")
         w("        const int rep = rep0 + 64 * i;")
-        w(f"        if (rep < {self.B}) upd_{a}(A, S, S, (int64_t)rep, rep, own, x, y, z, key);")
+        # the launch covers replicas [rep_base, rep_base + block) (the whole batch when unblocked)
+        w(f"        if (rep - A.rep_base < {self.rm_block}) upd_{a}(A, S, S, (int64_t)rep, rep, own, x, y, z, key);")
         w("    }")
         w("}")
         w()
@@ -902,7 +920,7 @@ class _Gen:
         site's scalar work, fewer keep the workgroups in flight on an XCD within its L2."""
         if self.rm_R is not None:
             return self.rm_R
-        return max(1, min(4, self.B // BLOCK))
+        return max(1, min(4, self.rm_block // BLOCK))
 
     def emit_residue(self, c: int, a: int) -> None:
         """x0: the x residue in [0, P) of row (y, z) where sublattice a has colour c (P when none)."""
@@ -1634,6 +1652,7 @@ class _Gen:
         R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
         gy, gz = (R, 1) if R <= 65535 else (65535, -(-R // 65535))
         sweep = []
+        nblk = self.B // self.rm_block if self.replica_major else 1
         for k in range(len(self.classes)):
             if self.mode == MODE_PACKED:
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // (BLOCK // 32)), self.W, 1)))
@@ -1660,8 +1679,11 @@ class _Gen:
                Launch(ki["sck_obs_final"], (-(-self.B // 64), 1, 1), (64, 1, 1))]
         bump = Launch(ki["sck_bump"], (1, 1, 1), (64, 1, 1))
         layouts, offsets, total = ising_layout(s) if s.n == 1 else ([], [], 0)
+        if nblk > 1:
+            # block-major: the class launches of replica block 0, then of block 1, ...
+            sweep = [replace(l, rep_base=rb * self.rm_block) for rb in range(nblk) for l in sweep]
         mod = Module(src, kernels, sweep, ref, sync, obs, bump, self.obs_chunks, self.classes,
-                     table_entries=total, ising_layout=layouts)
+                     table_entries=total, ising_layout=layouts, rep_blocks=nblk)
         if rplan is not None:
             mod.resident = Launch(ki["sck_resident"], (self.B, 1, 1), (self.RESIDENT_THREADS, 1, 1))
         mod.key = hashlib.sha256(src.encode()).hexdigest()[:16]
@@ -1886,7 +1908,7 @@ def fits_resident(sys_: System, mode: int) -> bool:
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
              replica_major: bool = False, rm_tile=None, rm_chunks=None, rm_sites=None,
-             resident_threads=None) -> Module:
+             resident_threads=None, rm_block=None) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
@@ -1894,4 +1916,4 @@ def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", lea
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
     return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites,
-                resident_threads).generate()
+                resident_threads, rm_block).generate()

@@ -8,7 +8,9 @@ and a Sampler on a non-HIP device raises.
 Additions: `proposal="cone"` (fixed half-angle per replica, `set_width`) and `proposal="learned"` (a width table
 from sck_amd.proposal) for XY/Heisenberg, and `observe(S)` returning energy and magnetization from one kernel.
 Whole sweeps of a system whose per-replica lattice fits in LDS run in one launch of the resident kernel
-(`resident=False` keeps the per-class launches).
+(`resident=False` keeps the per-class launches). A replica-major batch whose lattice exceeds the cache runs
+in replica blocks, each through all its sweeps before the next (`replica_block`: replicas per block, a
+multiple of 64 dividing the batch; 0 turns blocks off; default automatic).
 The Philox counter is 64-bit (identical to the reference's int32 counter below 2^31), so plain batches beyond
 the reference's limit run; the bitwise kernels keep the int32 counter w * nsite + i and its limit.
 """
@@ -89,7 +91,7 @@ class Sampler:
                  words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
                  bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
                  width=None, learned=None, cache_dir=None, resident: bool = True, layout: str = "auto",
-                 tile=None, chunks=None, sites_per_block=None, resident_threads=None):
+                 tile=None, chunks=None, sites_per_block=None, resident_threads=None, replica_block=None):
         self.sys = system
         self.use_resident = bool(resident)
         self.bitwise = bool(bitwise)
@@ -159,7 +161,8 @@ class Sampler:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta,
                                  replica_major=self.replica_major, rm_tile=tile, rm_chunks=chunks,
-                                 rm_sites=sites_per_block, resident_threads=resident_threads)
+                                 rm_sites=sites_per_block, resident_threads=resident_threads,
+                                 rm_block=replica_block)
         self.src = self.module.src
         dev = self.device
         m = system.site_mask()
@@ -212,6 +215,7 @@ class Sampler:
         d.mode = self.mode
         d.n, d.b, d.L1, d.L2, d.L3 = self.sys.n, self.sys.b, self.sys.L1, self.sys.L2, self.sys.L3
         d.batch, d.words, d.obs_chunks = self.B, self.W, mod.obs_chunks
+        d.rep_blocks = mod.rep_blocks
         d.kernel_names = C.cast(names, C.POINTER(C.c_char_p))
         d.n_kernels = len(mod.kernels)
         for field_, plan in (("sweep", mod.sweep), ("ref", mod.ref), ("sync", mod.sync), ("obs", mod.obs)):

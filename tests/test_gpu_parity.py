@@ -339,6 +339,40 @@ def test_graph_replay_equals_eager_sweeps_and_reads_beta_by_address():
     assert int(A.state[1]) == 12
 
 
+@pytest.mark.parametrize("make, B, blk, chunks", [
+    (lambda: sy.System("Heis bcc J1J2 D h", 3, cr.make("bcc", (1, 2)), (4, 4, 4), (True,) * 3, {1: 1.0, 2: -0.2},
+                       h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0))), 256, 64, None),
+    (lambda: sy.xy("triangular", 12), 512, 128, 2)])
+def test_replica_blocks_equal_unblocked_sweeps(make, B, blk, chunks):
+    """Replica blocks (each block through all n sweeps, then the next) leave every replica's chain unchanged:
+    whole sweeps, a class range and graph replays give the spins of the unblocked sampler."""
+    s = make()
+    print(f"[gpu] replica blocks {s.name} B={B} block={blk}", flush=True)
+    P = Sampler(s, B, DEV, seed=9, layout="replica", replica_block=blk, chunks=chunks)
+    Q = Sampler(s, B, DEV, seed=9, layout="replica", replica_block=0, chunks=chunks)
+    assert P.module.rep_blocks == B // blk and Q.module.rep_blocks == 1
+    T = torch.linspace(0.3, 3.0, B)
+    for X in (P, Q):
+        X.set_temperature(T)
+    S1, S2 = P.init_spins(seed=4), Q.init_spins(seed=4)
+    P.sweep(S1, 3)
+    Q.sweep(S2, 3)
+    assert torch.equal(S1, S2)
+    P.sweep(S1, 2, classes=(1, 3))
+    Q.sweep(S2, 2, classes=(1, 3))
+    assert torch.equal(S1, S2)
+    rep = P.graph(S1, 2)
+    for _ in range(2):
+        rep()
+        Q.sweep(S2, 2)
+        torch.cuda.synchronize()
+        assert torch.equal(S1, S2)
+    ms, nl = P.sweep_timed(S1, 1)
+    Q.sweep(S2, 1)
+    assert nl == len(P.classes) * (B // blk) and torch.equal(S1, S2)
+    assert int(P.state[1]) == int(Q.state[1]) == 10
+
+
 @pytest.mark.parametrize("resident", [False, True])
 def test_timed_sweeps_equal_plain_sweeps(resident):
     """sck_sweep_timed runs the same sweeps (spins and counter) and counts one event pair per sweep-kernel

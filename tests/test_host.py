@@ -145,7 +145,8 @@ def test_sampler_refuses_cpu_device():
 
 
 @pytest.mark.parametrize("case", ["bcc32-8192", "tri64-2048", "sq30-192-tile", "sc10-320-tile", "bcc32-8192-R1",
-                                  "bcc32-8192-R2", "bcc32-8192-site-waves", "sq30-192-site-waves"])
+                                  "bcc32-8192-R2", "bcc32-8192-site-waves", "sq30-192-site-waves",
+                                  "bcc4-256-block64", "bcc4-512-block128-site-waves"])
 def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
     """The launch plan's grid with the kernel's decode (XCD share -> (replica group, tiled slot)) visits every
     (group, class slot) exactly once; ragged tiles and the padded XCD shares fall out of range."""
@@ -161,22 +162,32 @@ def test_replica_major_grid_covers_every_site_and_replica_group_once(case):
                                               {1: 1.0, 2: -0.2}), 8192, (8, 4, 2), 2),
           "bcc32-8192-site-waves": (lambda: sy.System("b", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3,
                                                       {1: 1.0, 2: -0.2}), 8192, (4, 4, 4), 2, 4),
-          "sq30-192-site-waves": (lambda: sy.xy("square", 30), 192, (7, 5, 1), 1, 4)}[case]
+          "sq30-192-site-waves": (lambda: sy.xy("square", 30), 192, (7, 5, 1), 1, 4),
+          "bcc4-256-block64": (lambda: sy.heisenberg("bcc", 4), 256, None, None, 1, 64),
+          "bcc4-512-block128-site-waves": (lambda: sy.heisenberg("bcc", 4), 512, None, 2, 4, 128)}[case]
     s, B, tile = mk[0](), mk[1], mk[2]
     R = mk[3] if len(mk) > 3 else None
     SP = mk[4] if len(mk) > 4 else 1
-    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile, rm_chunks=R, rm_sites=SP)
-    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile, rm_chunks=R, rm_sites=SP)
+    blk = mk[5] if len(mk) > 5 else None
+    g = G._Gen(s, G.MODE_F32, B, "uniform", replica_major=True, rm_tile=tile, rm_chunks=R, rm_sites=SP,
+               rm_block=blk)
+    mod = G.generate(s, G.MODE_F32, B, replica_major=True, rm_tile=tile, rm_chunks=R, rm_sites=SP, rm_block=blk)
     Gn, S, chunk = g.rm_plan()
     tx, ty, tz = g.rm_tile
     ntx, nty = -(-g.BX // tx), -(-g.L2 // ty)
     gx, gy, gz = mod.sweep[0].grid
     assert gx == G.XCDS and gy * gz >= chunk
-    # replicas of a group: every replica of the batch exactly once over the groups
+    # replicas: each launch covers its block [rep_base, rep_base + block) through its groups (lanes past the
+    # block's end skip the update), and the blocks cover every replica of the batch exactly once
     Rc = g.rm_chunks()
     per = 64 * Rc * (4 // SP)
-    reps = np.concatenate([grp * per + np.arange(per) for grp in range(Gn)])
-    assert np.array_equal(np.sort(reps[reps < B]), np.arange(B))
+    Bb = g.rm_block
+    bases = sorted({l.rep_base for l in mod.sweep})
+    assert bases == list(range(0, B, Bb)) and len(mod.sweep) == len(bases) * len(mod.classes)
+    reps = np.concatenate([b0 + r for b0 in bases for r in [np.concatenate([grp * per + np.arange(per)
+                                                                             for grp in range(Gn)])]
+                           for r in [r[r < Bb]]])
+    assert np.array_equal(np.sort(reps), np.arange(B))
     seen = np.zeros((Gn, g.L3, g.L2, g.BX), np.int64)
     for bx in range(gx):
         k = np.arange(gy * gz)
@@ -285,3 +296,26 @@ def test_specialised_philox_equals_general_and_oracle(tmp_path):
     for i in range(len(c0)):
         ref = O.philox((int(k0[i]), 0), [[int(c0[i]), 0, 0, 0]])[0]
         assert np.array_equal(out[i, :4], ref.astype(np.uint64)), i
+
+
+def test_replica_blocks_plan():
+    """A replica-major lattice beyond the cache budget runs in replica blocks (C3: 32 blocks of 256 replicas,
+    201 MB each) with the sweep list block-major; smaller lattices and blocks below one wave stay unblocked;
+    a forced block must be a multiple of 64 dividing the batch."""
+    s = sy.System("Heisenberg bcc J1J2", 3, cr.make("bcc", (1, 2)), (32, 32, 32), (True,) * 3, {1: 1.0, 2: -0.2},
+                  h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0)))
+    m = hg.generate(s, hg.MODE_F32, 8192, replica_major=True)
+    ncls = len(m.classes)
+    assert m.rep_blocks == 32 and len(m.sweep) == 32 * ncls
+    assert [l.rep_base for l in m.sweep] == [256 * (i // ncls) for i in range(len(m.sweep))]
+    assert [l.kernel for l in m.sweep[:ncls]] == [l.kernel for l in m.sweep[ncls:2 * ncls]]
+    assert 256 * s.nsite * 3 * 4 <= hg.RM_BLOCK_BYTES < 512 * s.nsite * 3 * 4
+    assert hg.generate(s, hg.MODE_F32, 128, replica_major=True).rep_blocks == 1
+    assert hg.generate(s, hg.MODE_F32, 8192, replica_major=True, rm_block=0).rep_blocks == 1
+    assert hg.generate(sy.xy("triangular", 2048), hg.MODE_F32, 2048, replica_major=True).rep_blocks == 1
+    small = sy.heisenberg("bcc", 4)
+    f = hg.generate(small, hg.MODE_F32, 256, replica_major=True, rm_block=64)
+    assert f.rep_blocks == 4 and "A.rep_base +" in f.src
+    for bad in (32, 96, 100):
+        with pytest.raises(ValueError, match="replica block"):
+            hg.generate(small, hg.MODE_F32, 256, replica_major=True, rm_block=bad)
