@@ -523,8 +523,15 @@ class _Gen:
                 w("        int g = 0;")
                 for i in members:
                     q, ok = self.nb_index(i)
-                    w(f"        {{ const int q = {q}; const int t = ({ok}) ? (int){self.at('src', 0, 'q')} : 0; g += t;"
-                      + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
+                    if ok == "true":
+                        w(f"        {{ const int q = {q}; const int t = (int){self.at('src', 0, 'q')}; g += t;"
+                          + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
+                    else:
+                        # outside the box: read the site itself (always in range) and use 0, so the loads
+                        # issue without branches
+                        w(f"        {{ const bool ok = {ok}; const int q = ok ? ({q}) : own; "
+                          f"const int r = (int){self.at('src', 0, 'q')}; const int t = ok ? r : 0; g += t;"
+                          + (f" ext{extras.index(i)} = t;" if i in extras else "") + " }")
                 if Jv != 0.0:
                     w(f"        gsum{gi} = g + {len(members)};")
                     gi += 1
@@ -621,9 +628,18 @@ class _Gen:
                 w(f"{ind}    float g{k} = 0.0f;")
             for i in members:
                 q, ok = self.nb_index(i)
-                w(f"{ind}    {{ const int q = {q}; const bool ok = {ok};")
+                if ok == "true":
+                    w(f"{ind}    {{ const int q = {q};")
+                else:
+                    # outside the box: read the site itself (always in range) and use 0, so the neighbour loads
+                    # issue together without branches
+                    w(f"{ind}    {{ const bool ok = {ok}; const int q = ok ? ({q}) : own;")
                 for k in range(n):
-                    w(f"{ind}      const float t{k} = ok ? (float){self.at('src', k, 'q')} : 0.0f; g{k} = g{k} + t{k};")
+                    if ok == "true":
+                        w(f"{ind}      const float t{k} = (float){self.at('src', k, 'q')}; g{k} = g{k} + t{k};")
+                    else:
+                        w(f"{ind}      const float r{k} = (float){self.at('src', k, 'q')};")
+                        w(f"{ind}      const float t{k} = ok ? r{k} : 0.0f; g{k} = g{k} + t{k};")
                     if i in extras:
                         w(f"{ind}      n{extras.index(i)}_{k} = t{k};")
                 w(f"{ind}    }}")
