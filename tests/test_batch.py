@@ -325,6 +325,12 @@ def test_batch_parallel_sweep_equals_its_sequential_chain_and_graph_replay():
     rep()
     torch.cuda.synchronize()
     assert torch.equal(S1, S3) and int(Bt.state[1]) == 2
+    # timed sweeps: the same chain, one event pair per class launch
+    Bt.state[1] = 0
+    S4 = S0.clone()
+    ms, nl = Bt.sweep_timed(S4, 2)
+    assert torch.equal(S1, S4) and int(Bt.state[1]) == 2
+    assert nl == 2 * len(Bt.classes) and ms > 0.0
 
 
 @pytest.mark.gpu
