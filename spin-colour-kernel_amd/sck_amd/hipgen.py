@@ -1289,6 +1289,37 @@ class _Gen:
         w("}")
         w()
 
+    def emit_count_planes(self, xs: list, NP: int) -> None:
+        """c0 .. c{NP-1}: the bit-planes of the per-lane count of set bits across the planes `xs` (count <
+        2^NP), by a carry-save tree of full adders (3-input xor and majority, one bitop3 each) and half adders,
+        column by column from the lowest weight."""
+        w = self.w
+        cols = [list(xs)] + [[] for _ in range(NP)]
+        tmp = 0
+        for wt in range(NP):
+            col = cols[wt]
+            while len(col) >= 3:
+                a, b, c = col.pop(0), col.pop(0), col.pop(0)
+                sname, cname = f"f{tmp}s", f"f{tmp}c"
+                tmp += 1
+                # truth tables over (a, b, c) = (0xF0, 0xCC, 0xAA): a ^ b ^ c = 0x96, majority = 0xE8
+                w(f"    const uint32_t {sname} = __builtin_amdgcn_bitop3_b32({a}, {b}, {c}, 0x96);")
+                if wt + 1 < NP:
+                    w(f"    const uint32_t {cname} = __builtin_amdgcn_bitop3_b32({a}, {b}, {c}, 0xE8);")
+                    cols[wt + 1].append(cname)
+                col.append(sname)
+            if len(col) == 2:
+                a, b = col.pop(0), col.pop(0)
+                sname, cname = f"f{tmp}s", f"f{tmp}c"
+                tmp += 1
+                w(f"    const uint32_t {sname} = {a} ^ {b};")
+                if wt + 1 < NP:
+                    w(f"    const uint32_t {cname} = {a} & {b};")
+                    cols[wt + 1].append(cname)
+                col.append(sname)
+            # the count is below 2^NP, so no carry leaves the top plane
+            w(f"    const uint32_t c{wt} = {col[0] if col else '0u'};")
+
     def bw_entries(self, a: int):
         """Flip-mask entries of the independent mode, one per k (and per spin sign with a field), as
         (kind, lane-mask expression, table offset): 'static' entries (Metropolis on a periodic unmasked box
@@ -1341,18 +1372,18 @@ class _Gen:
             if indep:
                 w("    *s_out = s0;")
             w("    int ze = 0;")
-            for p in range(NP):
-                w(f"    uint32_t c{p} = 0u;")
-            for i in self.bw_bonds(a):
+            # anti-aligned present neighbours, one bit-plane each; their count goes into NP bit-planes c_p
+            xs = []
+            for j, i in enumerate(self.bw_bonds(a)):
                 q, ok = self.nb_index(i)
                 mk = " && A.mask[q]" if self.has_mask else ""
-                w(f"    {{ const int q = {q}; if (({ok}){mk}) {{ ze += 1; uint32_t cy = s0 ^ {self.word('Sw', 'q')};")
-                for p in range(NP):
-                    if p < NP - 1:
-                        w(f"        const uint32_t t{p} = c{p} & cy; c{p} ^= cy; cy = t{p};")
-                    else:
-                        w(f"        c{p} ^= cy;")
-                w("    } }")
+                if ok == "true" and not mk:
+                    w(f"    ze += 1; const uint32_t x{j} = s0 ^ {self.word('Sw', f'({q})')};")
+                else:
+                    w(f"    uint32_t x{j} = 0u; {{ const int q = {q}; if (({ok}){mk}) {{ ze += 1; "
+                      f"x{j} = s0 ^ {self.word('Sw', 'q')}; }} }}")
+                xs.append(f"x{j}")
+            self.emit_count_planes(xs, NP)
             eqs = []
             for k in range(Z1):
                 terms = [f"c{p}" if (k >> p) & 1 else f"~c{p}" for p in range(NP)]
