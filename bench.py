@@ -324,10 +324,16 @@ def main() -> None:
     if a.calibrate_copy:
         x = torch.ones(2 ** 28, dtype=torch.float32, device="cuda:0")
         y = torch.empty_like(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        y.copy_(x)
+        e0.record()
         for _ in range(3):
             y.copy_(x)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "copies": 3}))
+        ms = e0.elapsed_time(e1) / 3
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "copies": 4,
+                          "ms_per_copy": ms, "copy_GBps": 2 * x.numel() * 4 / (ms / 1e3) / 1e9}))
         return
     if a.workload == "learned":
         learned_main(a)
