@@ -1498,9 +1498,6 @@ class _Gen:
                 w(f"    {self.word('Sw', 'own', const=False)} ^= flip_{a}(A, S, wd, own, x, y, z, key);")
             w("}")
             w()
-        if indep:
-            # a dead lane must not decide anything: make its flip mask computation harmless
-            pass
         w(f'extern "C" __global__ void __launch_bounds__(64) sck_seq(sck_args A) {{')
         w("    const int wd = blockIdx.x * blockDim.x + threadIdx.x;")
         w("    if (wd >= A.nrep) return;")
