@@ -971,7 +971,7 @@ class _Gen:
         sweeps (barriers between classes replace kernel boundaries; replicas are independent chains, so no
         workgroup waits for another). Ising: a site whose table value decides it makes no draw; the others draw
         at once, so one barrier per class pass suffices (queueing them for dense draws cost a second barrier and
-        was 25 % slower at the 64^2 x 512 bench shape)."""
+        was 20 % slower at the 64^2 x 512 bench shape)."""
         w = self.w
         T = "int8_t" if self.mode == MODE_INT8 else "float"
         RT = self.RESIDENT_THREADS
