@@ -319,3 +319,15 @@ def test_replica_blocks_plan():
     for bad in (32, 96, 100):
         with pytest.raises(ValueError, match="replica block"):
             hg.generate(small, hg.MODE_F32, 256, replica_major=True, rm_block=bad)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 prints a JSON error line and exits with status 2 (before any
+    device use)."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
