@@ -40,7 +40,30 @@ WORKLOADS = {
     "nanodisks": "64 Heisenberg sc nanodisks, radius U[8,40], thickness U{4..12} (numpy default_rng(2024)), open, "
                  "J1=1, bulk DMI D=0.1, Ks=0.3 along the surface normal, 16 replicas each (T in [0.3,1.8]), fp32, "
                  "every disk in one launch per class, graph-captured sweeps, autotuned schedule",
+    "reference-table": "the reference's own throughput benchmark: its 13 systems at B = 8..128 and its 2-D bitwise "
+                       "scan, measured by its protocol, each next to its published H100 rate (one JSON line each)",
 }
+
+# Published H100 rates of the reference's throughput benchmark, flips/ns (= site updates per ns):
+# results-product/Results/tables/e6-throughput-h100.csv:2-14 (systems x B = 8, 16, 32, 64, 128) and
+# results-product/Results/tables/e6-scan.csv:2-5 ("Ours r10": bitwise independent, B = 64, side = 8 L).
+REF_H100 = {
+    "Ising square": (46.954, 57.2, 62.518, 67.982, 71.55),
+    "Ising triangular": (20.283, 24.666, 30.7, 35.838, 38.894),
+    "Ising sc": (19.349, 22.336, 50.054, 55.961, 59.688),
+    "Ising bcc": (38.318, 44.268, 91.613, 101.497, 108.281),
+    "Ising fcc": (39.567, 45.069, 75.4, 81.86, 85.83),
+    "XY square": (38.418, 45.008, 49.17, 50.606, 52.334),
+    "XY sc": (18.782, 21.942, 23.99, 25.159, 25.626),
+    "Heisenberg sc": (18.594, 21.786, 23.921, 24.886, 25.539),
+    "Heisenberg bcc": (37.083, 43.334, 46.689, 48.963, 50.691),
+    "Heisenberg fcc": (37.715, 42.585, 38.799, 40.888, 41.902),
+    "Heisenberg triangular AFM": (14.866, 17.204, 18.879, 19.578, 20.031),
+    "Heisenberg kagome AFM": (49.637, 58.17, 58.33, 56.66, 57.862),
+    "Heisenberg nanodot": (5.05, 8.14, 10.127, 12.025, 13.433),
+}
+REF_H100_BATCHES = (8, 16, 32, 64, 128)
+REF_H100_SCAN = {512: 572.8, 1024: 698.8, 2048: 745.2, 4096: 761.1}
 
 
 def _make(workload: str):
@@ -173,6 +196,83 @@ def traffic_from_profiles(workload: str):
     if f.exists():
         return json.loads(f.read_text()).get("bytes_per_launch")
     return None
+
+
+def _graph_rate(A, S, k: int, upd: float, reps: int) -> float:
+    """Median flips/ns over `reps` measurements of >= ~100 ms of replays of a graph of k sweeps, the
+    reference's protocol (SCK/exp_perf.py:471-505, _events_ms at :22-30)."""
+    import math
+
+    import numpy as np
+    import torch
+    rp = A.graph(S, k)
+    for _ in range(2):
+        rp()
+    torch.cuda.synchronize()
+    ms0 = rp.time_ms(1)
+    nrep = max(1, int(math.ceil(100.0 / ms0)))
+    return float(np.median([upd * nrep / (rp.time_ms(nrep) * 1e6) for _ in range(reps)]))
+
+
+def reference_table_main(a) -> None:
+    """The reference's throughput benchmark on this backend: the 13 systems of SCK/exp_perf.py:60-74 at
+    B = 8, 16, 32, 64, 128 with the Sampler's default options (packed storage for Ising from B = 32, as the
+    reference), T = the system's reference temperature, seed 1, graph replays of k = ceil(2e7 / (B N))
+    sweeps; then the 2-D bitwise scan of SCK/exp_perf.py:202-231 (Ising square L = side / 8, B = 64,
+    independent draws, T = Tc, seed 5011). One JSON line per row with the published H100 rate beside it,
+    then the geometric mean of the ratios."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from sck_amd import system as sy
+    from sck_amd.sampler import Sampler
+    systems = [
+        sy.ising("square", 256, T_ref=sy.TC["Ising square"]),
+        sy.ising("triangular", 192, T_ref=sy.TC["Ising triangular"]),
+        sy.ising("sc", 32, T_ref=sy.TC["Ising sc"]),
+        sy.ising("bcc", 32, T_ref=sy.TC["Ising bcc"]),
+        sy.ising("fcc", 32, T_ref=sy.TC["Ising fcc"]),
+        sy.xy("square", 256, T_ref=0.893),
+        sy.xy("sc", 32, T_ref=sy.TC["XY sc"]),
+        sy.heisenberg("sc", 32, T_ref=sy.TC["Heisenberg sc"]),
+        sy.heisenberg("bcc", 32, T_ref=sy.TC["Heisenberg bcc"]),
+        sy.heisenberg("fcc", 32, T_ref=1.0),
+        sy.heisenberg("triangular", 192, J=-1.0, T_ref=0.285, label="Heisenberg triangular AFM"),
+        sy.heisenberg("kagome", 192, J=-1.0, T_ref=0.1, label="Heisenberg kagome AFM"),
+        sy.nanodot(),
+    ]
+    reps = 5
+    ratios = []
+    for s in systems:
+        for B, ref in zip(REF_H100_BATCHES, REF_H100[s.name]):
+            A = Sampler(s, B, "cuda:0", 1)
+            A.set_temperature(s.T_ref or 1.0)
+            S = A.init_spins(seed=1)
+            k = max(1, int(math.ceil(2e7 / (B * s.n_active))))
+            r = _graph_rate(A, S, k, B * s.n_active * k, reps)
+            ratios.append(r / ref)
+            print(json.dumps({"benchmark": "reference throughput table", "system": s.name, "batch": B,
+                              "active_sites": s.n_active, "storage": A.layout if not A.packed else "packed",
+                              "flips_per_ns": r, "h100_flips_per_ns": ref, "ratio": r / ref}), flush=True)
+            del A, S
+            torch.cuda.empty_cache()
+    for L, ref in REF_H100_SCAN.items():
+        s = sy.ising("square", L, T_ref=sy.TC["Ising square"])
+        A = Sampler(s, 64, "cuda:0", 5011, bitwise=True, independent=True)
+        A.set_temperature(sy.TC["Ising square"])
+        S = A.init_spins(seed=5011)
+        k = max(2, int(2e10 // (64 * L * L)) // 10)
+        r = _graph_rate(A, S, k, 64 * L * L * k, reps)
+        ratios.append(r / ref)
+        print(json.dumps({"benchmark": "reference bitwise scan", "system": f"bitwise Ising square L={L}", "batch": 64,
+                          "flips_per_ns": r, "h100_flips_per_ns": ref, "ratio": r / ref}), flush=True)
+        del A, S
+        torch.cuda.empty_cache()
+    print(json.dumps({"benchmark": "reference throughput table and bitwise scan", "rows": len(ratios),
+                      "geomean_ratio_vs_h100": float(np.exp(np.mean(np.log(ratios)))),
+                      "min_ratio": float(min(ratios)), "max_ratio": float(max(ratios))}), flush=True)
 
 
 def learned_main(a) -> None:
@@ -337,6 +437,9 @@ def main() -> None:
         return
     if a.workload == "learned":
         learned_main(a)
+        return
+    if a.workload == "reference-table":
+        reference_table_main(a)
         return
     if a.workload == "nanodisks":
         nanodisk_main(a)

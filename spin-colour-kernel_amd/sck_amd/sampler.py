@@ -7,8 +7,9 @@ and a Sampler on a non-HIP device raises.
 
 Additions: `proposal="cone"` (fixed half-angle per replica, `set_width`) and `proposal="learned"` (a width table
 from sck_amd.proposal) for XY/Heisenberg, and `observe(S)` returning energy and magnetization from one kernel.
-Whole sweeps of a system whose per-replica lattice fits in LDS run in one launch of the resident kernel
-(`resident=False` keeps the per-class launches). A replica-major batch whose lattice exceeds the cache runs
+Whole sweeps of a system whose per-replica lattice fits in LDS run in one launch of the resident kernel when
+the batch fills the GPU (B >= 128) or the lattice is small (at most 8192 sites); `resident=False` keeps the
+per-class launches. A replica-major batch whose lattice exceeds the cache runs
 in replica blocks, each through all its sweeps before the next (`replica_block`: replicas per block, a
 multiple of 64 dividing the batch; 0 turns blocks off; default automatic).
 The Philox counter is 64-bit (identical to the reference's int32 counter below 2^31), so plain batches beyond
@@ -30,6 +31,12 @@ __all__ = ["Sampler", "COMPACT_MIN", "TILE", "MAX_BLOCK", "PTILE", "WORD", "BWG"
            "p_fixed"]
 
 COMPACT_MIN = 4
+# the resident kernel runs one workgroup per replica, so it needs a batch that fills the compute units, unless
+# the lattice is small enough that per-class launches are launch-bound anyway (measured on int8 Ising: at
+# B = 8 and 32 per-class launches are 2-3x faster for 32k-site lattices, the resident kernel 2x faster from
+# B = 128, and faster at every B for a 4096-site lattice)
+RESIDENT_MIN_BATCH = 128
+RESIDENT_SMALL_SITES = 8192
 TILE = 128
 MAX_BLOCK = 1024
 PTILE = 64
@@ -118,6 +125,7 @@ class Sampler:
         self.W = -(-self.B // WORD) if self.packed else self.B
         self.Bp = self.W * WORD if self.packed else self.B
         nsite = b * L3 * L2 * L1
+        self.use_resident = self.use_resident and (self.B >= RESIDENT_MIN_BATCH or nsite <= RESIDENT_SMALL_SITES)
         if self.bitwise and self.W * nsite >= 2 ** 31 - 1:
             raise ValueError("replica batch too large for the int32 Philox counter")
         gmax = words_per_program or BWG
