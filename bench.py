@@ -40,6 +40,9 @@ WORKLOADS = {
     "nanodisks": "64 Heisenberg sc nanodisks, radius U[8,40], thickness U{4..12} (numpy default_rng(2024)), open, "
                  "J1=1, bulk DMI D=0.1, Ks=0.3 along the surface normal, 16 replicas each (T in [0.3,1.8]), fp32, "
                  "every disk in one launch per class, graph-captured sweeps, autotuned schedule",
+    "learned-xy": "XY triangular L=2048 periodic J1=1, B=512 (16 temps x 32 in [0.7,1.1]; the 2048-replica batch "
+                  "would need four 69 GB copies of the lattice): effective samples/s of the learned width over "
+                  "the best of five fixed widths, tau_int of E and |M| per temperature",
     "reference-table": "the reference's own throughput benchmark: its 13 systems at B = 8..128 and its 2-D bitwise "
                        "scan, measured by its protocol, each next to its published H100 rate (one JSON line each)",
 }
@@ -279,17 +282,26 @@ def learned_main(a) -> None:
     """Effective independent samples/s of E and |M|, learned width vs the best fixed width (one JSON line)."""
     import numpy as np
 
+    from sck_amd import system as sy
     from sck_amd.ess import compare_proposals
-    s, _, B, T, _ = _make("heisenberg")
-    temps = np.unique(T.numpy())
+    if a.workload == "learned-xy":
+        s, B = sy.xy("triangular", 2048), 512
+        temps = np.linspace(0.7, 1.1, 16).astype(np.float32)
+        equil = 300
+    else:
+        s, _, B, T, _ = _make("heisenberg")
+        temps = np.unique(T.numpy())
+        equil = 1000
     meas = max(50, a.steps)
-    r = compare_proposals(s, B, temps, warmup=200, equil=1000, therm=max(20, a.warmup), n_meas=meas,
-                          log=lambda m: print(m, flush=True))
+    print(f"comparing proposals on {s.name}, B={B}", flush=True)
+    r = compare_proposals(s, B, temps, warmup=200, equil=equil, therm=max(20, a.warmup), n_meas=meas,
+                          log=lambda m: print(m, flush=True),
+                          init=(lambda A: device_spins(A, seed=0)) if a.workload == "learned-xy" else None)
     out = {"metric": "effective samples/s, learned vs fixed proposal", "value": r["gain_E"],
            "unit": "ratio of ESS/s of the energy, geometric mean over the temperature ladder", "n_gpus": 1, "steps": meas, "warmup": a.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (random initial spins from the seeded torch CPU generator)",
-           "config": {"workload": WORKLOADS["learned"], "batch": B, "sites": s.n_active},
+           "config": {"workload": WORKLOADS[a.workload], "batch": B, "sites": s.n_active},
            "gain_E": r["gain_E"], "gain_M": r["gain_M"], "gain_E_bottleneck": r["gain_E_bottleneck"],
            "gain_M_bottleneck": r["gain_M_bottleneck"], "gain_E_sum": r["gain_E_sum"], "gain_M_sum": r["gain_M_sum"],
            "fixed_width": r["fixed_width"], "fixed_scan_gm_ess_per_s": r["fixed_scan_gm_ess"],
@@ -298,10 +310,14 @@ def learned_main(a) -> None:
            "ess_per_s": {k: {"E": r[k]["ess_E_per_s"], "M": r[k]["ess_M_per_s"],
                              "updates_per_s": r[k]["updates_per_s"]} for k in ("uniform", "fixed", "learned")},
            "observables_max_z": r["max_z"], "observables_frac_within_2sigma": r["frac_within_2sigma"],
+           "temperatures": [float(t) for t in temps],
+           "tau_int": {k: {"E": [float(r[k]["tau_E"][j]) for j in range(len(temps))],
+                           "M": [float(r[k]["tau_M"][j]) for j in range(len(temps))]}
+                       for k in ("uniform", "fixed", "learned")},
            "train_seconds": r["train_seconds"]}
     f = REPO / "gpurun_out"
     f.mkdir(exist_ok=True)
-    (f / "learned_comparison.json").write_text(json.dumps(r, default=float, indent=1))
+    (f / f"{a.workload.replace('-', '_')}_comparison.json").write_text(json.dumps(r, default=float, indent=1))
     print(json.dumps(out, default=float), flush=True)
 
 
@@ -435,7 +451,7 @@ def main() -> None:
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "copies": 4,
                           "ms_per_copy": ms, "copy_GBps": 2 * x.numel() * 4 / (ms / 1e3) / 1e9}))
         return
-    if a.workload == "learned":
+    if a.workload in ("learned", "learned-xy"):
         learned_main(a)
         return
     if a.workload == "reference-table":

@@ -66,7 +66,7 @@ def _zscores(a: dict, b: dict) -> np.ndarray:
 
 def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 1000, therm: int = 50,
                       n_meas: int = 1000, every: int = 1, seed: int = 0, train_steps: int = 400, log=print,
-                      widths=(0.5, 1.0, 1.5, 2.0, 2.5)) -> dict:
+                      widths=(0.5, 1.0, 1.5, 2.0, 2.5), init=None) -> dict:
     """Train the width on `warmup` fixed-width sweeps, equilibrate one state for `equil` sweeps with the
     learned width, then from that common equilibrium state run `therm` + `n_meas` sweeps of each of: the
     uniform redraw (the reference's proposal), each fixed width of `widths`, and the learned width. The
@@ -79,7 +79,9 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     # warm-up with a moderate fixed width; its records train the network and pick the fixed baseline
     W = Sampler(system, B, "cuda", seed=seed, proposal="cone", width=1.0)
     W.set_temperature(torch.as_tensor(T_rep))
-    S0 = W.init_spins(seed=seed)
+    # init(sampler) -> spins: e.g. drawn on the device for lattices too large for the host generator
+    S0 = W.init_spins(seed=seed) if init is None else init(W)
+    log(f"  initial state ready ({B} replicas x {int(N)} sites); warm-up and training")
     t0 = time.perf_counter()
     lw = train_width(W, S0, warmup=warmup, steps=train_steps, log=log)
     t_train = time.perf_counter() - t0

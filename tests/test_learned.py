@@ -139,11 +139,13 @@ def test_training_raises_the_objective_and_beats_the_best_fixed_width():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("model", ["heisenberg", "xy"])
 @pytest.mark.parametrize("kind", ["cone", "learned"])
-def test_cone_and_learned_decisions_match_oracle(kind):
+def test_cone_and_learned_decisions_match_oracle(kind, model):
     from sck_amd.proposal import LearnedWidth, init_params, mlp_config
     from sck_amd.sampler import Sampler
-    s = _heis_system()
+    s = _heis_system() if model == "heisenberg" else sy.xy("triangular", 12)
+    print(f"[gpu] {kind} decisions {s.name}", flush=True)
     B = 16
     kw = {}
     if kind == "cone":
@@ -204,6 +206,40 @@ def test_single_spin_distribution_is_exact(kind):
     x = np.linspace(-1, 1, 200001)
     wgt = np.exp((h * x + D * x * x) / T)
     exact = (x * wgt).sum() / wgt.sum()
+    est = float(np.mean(acc))
+    err = float(np.std(acc)) / math.sqrt(len(acc)) * 3 + 2e-3
+    assert abs(est - exact) < err, (kind, est, exact)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["uniform", "cone", "learned"])
+def test_single_xy_spin_distribution_is_exact(kind):
+    """One XY spin in a field h along x: <s_x> = I1(beta h) / I0(beta h) (modified Bessel functions); every
+    proposal must sample it."""
+    from scipy.special import iv
+    from sck_amd.proposal import LearnedWidth, init_params, mlp_config
+    from sck_amd.sampler import Sampler
+    h, T = 0.8, 0.5
+    s = sy.System("single XY spin", 2, cr.make("sc"), (1, 1, 1), (False,) * 3, {1: 1.0}, h=(h, 0.0))
+    B = 4096
+    kw = {}
+    if kind == "cone":
+        kw = dict(proposal="cone", width=0.9)
+    elif kind == "learned":
+        probe = Sampler(s, 64, "cuda", seed=1, proposal="cone", width=1.0)
+        cfg = mlp_config(probe)
+        cfg["xs"] = (0.5, 1.0, 1.0)
+        lw = LearnedWidth(init_params(6), cfg, n=(16, 4, 4), s=(4.0, 2.0, 4.0))
+        kw = dict(proposal="learned", learned=lw)
+    A = Sampler(s, B, "cuda", seed=9, **kw)
+    A.set_temperature(T)
+    S = A.init_spins(seed=1)
+    A.sweep(S, 200)
+    acc = []
+    for _ in range(40):
+        A.sweep(S, 5)
+        acc.append(S[:, 0].double().mean().item())
+    exact = float(iv(1, h / T) / iv(0, h / T))
     est = float(np.mean(acc))
     err = float(np.std(acc)) / math.sqrt(len(acc)) * 3 + 2e-3
     assert abs(est - exact) < err, (kind, est, exact)
