@@ -331,3 +331,19 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_reference_table_rows_cover_the_published_systems():
+    """The systems bench.py's reference-table workload builds carry the names of the published rows, each with
+    a rate per batch size (SCK/exp_perf.py:60-74; e6-throughput-h100.csv:2-14)."""
+    import sys
+    sys.path.insert(0, str(REPO))
+    import bench
+    names = [sy.ising("square", 8).name, sy.ising("triangular", 6).name, sy.ising("sc", 4).name,
+             sy.ising("bcc", 4).name, sy.ising("fcc", 4).name, sy.xy("square", 8).name, sy.xy("sc", 4).name,
+             sy.heisenberg("sc", 4).name, sy.heisenberg("bcc", 4).name, sy.heisenberg("fcc", 4).name,
+             sy.heisenberg("triangular", 6, J=-1.0, label="Heisenberg triangular AFM").name,
+             sy.heisenberg("kagome", 6, J=-1.0, label="Heisenberg kagome AFM").name, sy.nanodot(Rd=3.0, L=3).name]
+    assert sorted(names) == sorted(bench.REF_H100)
+    assert all(len(v) == len(bench.REF_H100_BATCHES) for v in bench.REF_H100.values())
+    assert sorted(bench.REF_H100_SCAN) == [512, 1024, 2048, 4096]
