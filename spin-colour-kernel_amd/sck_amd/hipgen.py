@@ -595,10 +595,12 @@ class _Gen:
             w(f"      dE = dE + {_hexf(Du)} * (se * se - te * te); }}")
             del se, te
         if s.site_ku is not None:
-            w(f"    {{ const float ku = A.site_ku[own]; const float *ax = A.site_axis + (int64_t)own * {n};")
+            # only sites with a surface term read its axis: where ku == 0 the term adds a signed zero, which
+            # leaves exp(-beta dE) (and Barker's 1 + exp(beta dE)) unchanged
+            w(f"    {{ const float ku = A.site_ku[own]; if (ku != 0.0f) {{ const float *ax = A.site_axis + (int64_t)own * {n};")
             w(f"      const float sn = {self._lassoc([f's{k} * ax[{k}]' for k in range(n)])};"
               f" const float tn = {self._lassoc([f't{k} * ax[{k}]' for k in range(n)])};")
-            w("      dE = dE + ku * (sn * sn - tn * tn); }")
+            w("      dE = dE + ku * (sn * sn - tn * tn); } }")
         w("    const float beta = A.beta[rep];")
         if s.acceptance == "barker":
             w("    const int acc = (u0 * (1.0f + exp_fast(beta * dE))) < 1.0f;")
