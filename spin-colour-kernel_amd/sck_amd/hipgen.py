@@ -53,6 +53,7 @@ RM_WAVES = 7                  # occupancy cap of the replica-major class kernels
 WORD = 32
 GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
 BW_QUEUE_DIV = 4              # independent bitwise mode: the tail queue holds 1/4 of a block's items
+BW_WORDS = 2                  # independent bitwise mode: words per thread, sharing the site's index work
 LEVELS = 32
 RM_BLOCK_BYTES = 224 << 20    # replica-major: largest lattice share run through whole sweeps at a time (cache-resident)
 
@@ -1173,8 +1174,12 @@ class _Gen:
 
     BW_ROUNDS = 4        # class slots per thread of an independent-mode block (fewer when queue items are large)
 
+    def bw_words(self) -> int:
+        """Words one thread of an independent-mode class kernel updates at each of its sites."""
+        return BW_WORDS if self.W % BW_WORDS == 0 else 1
+
     def bw_rounds(self, a: int) -> int:
-        item = 3 + 2 * max(1, self.bw_ndyn[a])
+        item = 4 + 2 * max(1, self.bw_ndyn[a])
         return max(1, min(self.BW_ROUNDS, (48 * 1024) // (BLOCK * item * 4)))
 
     def emit_bw_indep_class(self, k: int, c: int, a: int) -> None:
@@ -1182,22 +1187,24 @@ class _Gen:
         0 and 1 (two Philox calls: 8 of the 32 levels), which decide all 32 replicas of most (word, site)
         items; the few items still undecided are queued in LDS with their entries, and the block then runs
         the remaining level groups on dense waves of queued items. The draws and the result are those of the
-        plain loop: groups are taken in order and stop only when nothing is undecided."""
+        plain loop: groups are taken in order and stop only when nothing is undecided. A thread updates the
+        same site in bw_words() consecutive words, so the site's row decode and neighbour offsets are shared."""
         w = self.w
         nsite = self.NS
         nd = self.bw_ndyn[a]
-        item = 3 + 2 * max(1, nd)
+        WPT = self.bw_words()
+        item = 4 + 2 * max(1, nd)                     # own, acc, und, e[nd], P[nd], word
         R = self.bw_rounds(a)
-        qcap = max(64, R * BLOCK // BW_QUEUE_DIV)
+        qcap = max(64, R * BLOCK * WPT // BW_QUEUE_DIV)
+        iw = 3 + 2 * nd
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_class_{k}(sck_args A) {{')
         w(f"    __shared__ uint32_t q[{qcap}][{item}];")
         w("    __shared__ int qn;")
         w("    if (threadIdx.x == 0) qn = 0;")
         w("    __syncthreads();")
-        w("    const int wd = blockIdx.y;")
+        w(f"    const int wd0 = blockIdx.y * {WPT};")
         w("    const uint32_t key = sweep_key(A);")
         w(f"    uint32_t *S = (uint32_t *)A.spins;")
-        w(f"    uint32_t *Sw = S + (int64_t)wd * {nsite};")
         w("    const int lane = threadIdx.x & 63;")
         w(f"    for (int r = 0; r < {R}; ++r) {{")
         w(f"        const uint32_t t = (blockIdx.x * {R} + r) * blockDim.x + threadIdx.x;")
@@ -1218,30 +1225,36 @@ class _Gen:
         w(f"        const int own = live ? {a * self.PL} + row * {self.L1} + x : 0;")
         if self.has_mask:
             w("        live = live && A.mask[own];")
-        w("        uint32_t s0;")
+        w("#pragma unroll")
+        w(f"        for (int wi = 0; wi < {WPT}; ++wi) {{")
+        w("            const int wd = wd0 + wi;")
+        w(f"            uint32_t *Sw = S + (int64_t)wd * {nsite};")
+        w("            uint32_t s0;")
         # y and z stay in range whenever the row is (a dead lane's x is zeroed): with a wave-uniform row the
         # coordinates and their neighbour offsets stay scalar
-        w(f"        BW{a} st = setup_{a}(A, S, wd, own, live ? x : 0, rowok ? y : 0, rowok ? z : 0, &s0);")
-        w("        if (!live) { st.acc = 0u; st.und = 0u; }")
-        w(f"        const uint32_t ctr = (uint32_t)wd * {nsite}u + (uint32_t)own;")
-        w(f"        levels_{a}(st, ctr, key, 0, 2);")
-        w("        const bool tail = st.und != 0u;")
-        w(f"        if (live && !tail) {self.word('Sw', 'own', const=False)} = s0 ^ st.acc;")
-        w("        const unsigned long long m = __ballot(tail);")
-        w("        if (m) {")
-        w("            int qb = 0;")
-        w("            if (lane == 0) qb = atomicAdd(&qn, __popcll(m));")
-        w("            qb = __shfl(qb, 0, 64);")
-        w("            if (tail) {")
-        w("                const int pos = qb + __popcll(m & ((1ull << lane) - 1ull));")
-        w(f"                if (pos < {qcap}) {{")
-        w("                    q[pos][0] = (uint32_t)own; q[pos][1] = st.acc; q[pos][2] = st.und;")
+        w(f"            BW{a} st = setup_{a}(A, S, wd, own, live ? x : 0, rowok ? y : 0, rowok ? z : 0, &s0);")
+        w("            if (!live) { st.acc = 0u; st.und = 0u; }")
+        w(f"            const uint32_t ctr = (uint32_t)wd * {nsite}u + (uint32_t)own;")
+        w(f"            levels_{a}(st, ctr, key, 0, 2);")
+        w("            const bool tail = st.und != 0u;")
+        w(f"            if (live && !tail) {self.word('Sw', 'own', const=False)} = s0 ^ st.acc;")
+        w("            const unsigned long long m = __ballot(tail);")
+        w("            if (m) {")
+        w("                int qb = 0;")
+        w("                if (lane == 0) qb = atomicAdd(&qn, __popcll(m));")
+        w("                qb = __shfl(qb, 0, 64);")
+        w("                if (tail) {")
+        w("                    const int pos = qb + __popcll(m & ((1ull << lane) - 1ull));")
+        w(f"                    if (pos < {qcap}) {{")
+        w(f"                        q[pos][0] = (uint32_t)own; q[pos][1] = st.acc; q[pos][2] = st.und; "
+          f"q[pos][{iw}] = (uint32_t)wd;")
         for i in range(nd):
-            w(f"                    q[pos][{3 + i}] = st.e[{i}]; q[pos][{3 + nd + i}] = st.P[{i}];")
-        w("                } else {")
-        w("                    // queue full: finish the item here, with the same level groups in the same order")
-        w(f"                    levels_{a}(st, ctr, key, 2, {LEVELS // 4});")
-        w(f"                    {self.word('Sw', 'own', const=False)} = s0 ^ st.acc;")
+            w(f"                        q[pos][{3 + i}] = st.e[{i}]; q[pos][{3 + nd + i}] = st.P[{i}];")
+        w("                    } else {")
+        w("                        // queue full: finish the item here, with the same level groups in the same order")
+        w(f"                        levels_{a}(st, ctr, key, 2, {LEVELS // 4});")
+        w(f"                        {self.word('Sw', 'own', const=False)} = s0 ^ st.acc;")
+        w("                    }")
         w("                }")
         w("            }")
         w("        }")
@@ -1252,10 +1265,12 @@ class _Gen:
         w("        const bool has = i < n;")
         w(f"        BW{a} st;")
         w("        const int own = has ? (int)q[i][0] : 0;")
+        w(f"        const int wd = has ? (int)q[i][{iw}] : wd0;")
         w("        st.acc = has ? q[i][1] : 0u; st.und = has ? q[i][2] : 0u;")
         for i in range(nd):
             w(f"        st.e[{i}] = has ? q[i][{3 + i}] : 0u; st.P[{i}] = has ? q[i][{3 + nd + i}] : 0u;")
         w(f"        levels_{a}(st, (uint32_t)wd * {nsite}u + (uint32_t)own, key, 2, {LEVELS // 4});")
+        w(f"        uint32_t *Sw = S + (int64_t)wd * {nsite};")
         w(f"        if (has) {self.word('Sw', 'own', const=False)} ^= st.acc;")
         w("    }")
         w("}")
@@ -1674,7 +1689,8 @@ class _Gen:
                 sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // (BLOCK // 32)), self.W, 1)))
             elif self.mode == MODE_BITWISE_INDEP:
                 tile = BLOCK * self.bw_rounds(self.classes[k][1])
-                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // tile), self.W, 1)))
+                sweep.append(Launch(ki[f"sck_class_{k}"], (-(-self.ROWS * self.BX // tile),
+                                                            self.W // self.bw_words(), 1)))
             elif self.replica_major:
                 _, _, chunk = self.rm_plan()
                 gy = min(chunk, 65535)
