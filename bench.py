@@ -443,6 +443,8 @@ def main() -> None:
     ap.add_argument("--resident-threads", type=int, default=None, help="LDS-resident kernel: threads per replica")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference for the HBM counters)")
+    ap.add_argument("--calibrate-cache-copy", action="store_true",
+                    help="only run a plain 96 MiB device copy, whose source and destination stay in the Infinity Cache")
     ap.add_argument("--schedule", default="auto",
                     help="nanodisks: 'auto' (autotune) or kind/block/tile/order/grid, e.g. class/256/256/xcd/0")
     a = ap.parse_args()
@@ -466,6 +468,23 @@ def main() -> None:
         ms = e0.elapsed_time(e1) / 3
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "copies": 4,
                           "ms_per_copy": ms, "copy_GBps": 2 * x.numel() * 4 / (ms / 1e3) / 1e9}))
+        return
+    if a.calibrate_cache_copy:
+        # the plain copy on 96 MiB: source and destination (192 MiB) stay in the 256 MB Infinity Cache, beyond
+        # the 8 x 4 MB of L2, which is the level the replica-blocked and batched kernels stream from
+        x = torch.ones(24 << 20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(5):
+            y.copy_(x)
+        e0.record()
+        for _ in range(50):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 50
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "copies": 55,
+                          "ms_per_copy": ms, "cache_copy_GBps": 2 * x.numel() * 4 / (ms / 1e3) / 1e9}))
         return
     if a.workload in ("learned", "learned-xy"):
         learned_main(a)
