@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 #include <sys/stat.h>
+#include <unistd.h>
 
 // static kernels of the learned-proposal training (mlp.hip)
 extern "C" __global__ void sck_mlp_fwdbwd_k(const float *, const sck_mlp_sample *, int32_t, sck_mlp_config, float *);
@@ -76,7 +77,8 @@ bool read_file(const std::string &path, std::string &out) {
 }
 
 bool write_file_atomic(const std::string &path, const std::string &data) {
-    std::string tmp = path + ".tmp" + std::to_string((unsigned long long)fnv1a(data) & 0xffffff);
+    // a temporary name of this process's own, so that processes filling one cache never share a file
+    std::string tmp = path + ".tmp" + std::to_string((long long)::getpid());
     {
         std::ofstream f(tmp, std::ios::binary);
         if (!f) return false;
@@ -259,6 +261,9 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
     if (!d || !hip_src || !out) return fail(SCK_E_INVALID, "null argument");
     if (d->abi_version != SCK_ABI_VERSION) return fail(SCK_E_INVALID, "descriptor ABI version mismatch");
     if (d->n_kernels <= 0 || !d->kernel_names) return fail(SCK_E_INVALID, "no kernels");
+    if (!d->bump || d->n_sweep < 0 || d->n_ref < 0 || d->n_sync < 0 || d->n_obs < 0 ||
+        (d->n_sweep && !d->sweep) || (d->n_ref && !d->ref) || (d->n_sync && !d->sync) || (d->n_obs && !d->obs))
+        return fail(SCK_E_INVALID, "descriptor launch plans missing");
     if (d->rep_blocks < 1 || d->n_sweep % d->rep_blocks != 0)
         return fail(SCK_E_INVALID, "rep_blocks must be >= 1 and divide the sweep launches");
     std::string code;

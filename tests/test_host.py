@@ -125,6 +125,35 @@ def test_library_exports_every_declared_symbol():
     assert L.lib().sck_version() == version == L.SCK_ABI_VERSION
 
 
+def test_create_rejects_incomplete_descriptors_before_any_device_work():
+    """sck_create checks the descriptor before it compiles or loads anything: a missing counter-bump plan, a
+    plan count without its array, or a replica-block count that does not divide the sweep launches."""
+    from sck_amd import _lib as L
+    lib = L.lib()
+    names = (ctypes.c_char_p * 1)(b"sck_class_0")
+    plan, cnt = L.launches([hg.Launch(0, (1, 1, 1))])
+
+    def desc(**kw):
+        d = L.Desc()
+        d.abi_version = L.SCK_ABI_VERSION
+        d.kernel_names = ctypes.cast(names, ctypes.POINTER(ctypes.c_char_p))
+        d.n_kernels = 1
+        d.rep_blocks = 1
+        d.sweep = ctypes.cast(plan, ctypes.POINTER(L.LaunchC))
+        d.n_sweep = cnt
+        d.bump = ctypes.cast(plan, ctypes.POINTER(L.LaunchC))
+        for k, v in kw.items():
+            setattr(d, k, v)
+        return d
+
+    h = ctypes.c_void_p()
+    for d, msg in [(desc(bump=None), b"launch plans missing"), (desc(n_ref=2), b"launch plans missing"),
+                   (desc(rep_blocks=2), b"rep_blocks"), (desc(abi_version=L.SCK_ABI_VERSION + 1), b"ABI")]:
+        assert lib.sck_create(ctypes.byref(d), b"// unused", None, ctypes.byref(h)) != 0
+        assert msg in lib.sck_last_error()
+        assert not h.value
+
+
 def test_hiprtc_compiles_generated_modules_without_a_device(tmp_path):
     from sck_amd import _lib as L
     for s, mode in [(sy.heisenberg("bcc", 2), hg.MODE_F32), (sy.ising("sc", 4), hg.MODE_BITWISE_INDEP)]:
