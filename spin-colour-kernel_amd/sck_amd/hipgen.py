@@ -1225,15 +1225,23 @@ class _Gen:
         w(f"        const int own = live ? {a * self.PL} + row * {self.L1} + x : 0;")
         if self.has_mask:
             w("        live = live && A.mask[own];")
+        # the words' loads and neighbour counts first, before any divergent code: every word's loads keep the
+        # scalar-base form and are in flight together
+        w(f"        BW{a} stw[{WPT}];")
+        w(f"        uint32_t s0w[{WPT}];")
+        w("#pragma unroll")
+        w(f"        for (int wi = 0; wi < {WPT}; ++wi) {{")
+        # y and z stay in range whenever the row is (a dead lane's x is zeroed): with a wave-uniform row the
+        # coordinates and their neighbour offsets stay scalar
+        w(f"            stw[wi] = setup_{a}(A, S, wd0 + wi, own, live ? x : 0, rowok ? y : 0, rowok ? z : 0, &s0w[wi]);")
+        w("            if (!live) { stw[wi].acc = 0u; stw[wi].und = 0u; }")
+        w("        }")
         w("#pragma unroll")
         w(f"        for (int wi = 0; wi < {WPT}; ++wi) {{")
         w("            const int wd = wd0 + wi;")
         w(f"            uint32_t *Sw = S + (int64_t)wd * {nsite};")
-        w("            uint32_t s0;")
-        # y and z stay in range whenever the row is (a dead lane's x is zeroed): with a wave-uniform row the
-        # coordinates and their neighbour offsets stay scalar
-        w(f"            BW{a} st = setup_{a}(A, S, wd, own, live ? x : 0, rowok ? y : 0, rowok ? z : 0, &s0);")
-        w("            if (!live) { st.acc = 0u; st.und = 0u; }")
+        w("            const uint32_t s0 = s0w[wi];")
+        w(f"            BW{a} &st = stw[wi];")
         w(f"            const uint32_t ctr = (uint32_t)wd * {nsite}u + (uint32_t)own;")
         w(f"            levels_{a}(st, ctr, key, 0, 2);")
         w("            const bool tail = st.und != 0u;")
