@@ -255,6 +255,26 @@ def test_bitwise_against_oracle(independent, acceptance):
     assert torch.equal(S2, S)
 
 
+@pytest.mark.parametrize("h", [0.0, 0.3])
+def test_bitwise_whole_row_waves_against_oracle(h):
+    """Bitwise independent on a periodic square lattice whose class rows are whole waves (L = 128: 64 class
+    slots per row), so a wave's row, its liveness and its words' table rows are uniform and a thread updates
+    two words: every word against the oracle over 3 sweeps, with a field (which splits the entries by spin)
+    and without, a different temperature per word."""
+    s = sy.System("Ising square 128 h", 1, cr.make("square"), (128, 128), (True, True), {1: 1.0}, h=(h,))
+    B = 32 * 4
+    print(f"[gpu] bitwise whole-row waves h={h}", flush=True)
+    A = Sampler(s, B, DEV, seed=13, bitwise=True, independent=True)
+    A.set_temperature(torch.tensor([1.5, 2.27, 3.0, 5.0]).repeat_interleave(32))
+    S = A.init_spins(seed=6)
+    x = A.unpack(S).cpu().numpy()[:, 0]
+    tab = A.ptab.cpu().numpy()
+    for k in range(3):
+        A.sweep(S)
+        x = O.bitwise_chain(O.OracleSystem(s), x, tab, 13, k, 1, independent=True)
+        assert np.array_equal(A.unpack(S).cpu().numpy()[:, 0], x), f"sweep {k}"
+
+
 def test_bitwise_c2_shape_words_against_oracle():
     """Bitwise independent, sc 128^3 periodic, B = 1024 (32 words), T = 4.5: two sweeps on the GPU; words are
     independent chains, so the oracle follows words 0 and 31 over the whole lattice."""
