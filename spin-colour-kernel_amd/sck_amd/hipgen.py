@@ -49,6 +49,7 @@ __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT
 MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP = range(5)
 BLOCK = 256
 XCDS = 8                      # XCDs of an MI355X; consecutive workgroups are dispatched to them round-robin
+RESIDENT_GROUP = 4            # resident Ising kernel: sites a thread reads before it draws any of them
 RM_WAVES = 7                  # occupancy cap of the replica-major class kernels (waves per SIMD)
 WORD = 32
 GROUP_KEY_STEP = 104729       # independent bitwise mode: key of level group g (SCK/bitwise.py:159-168)
@@ -974,8 +975,9 @@ class _Gen:
         sweeps (barriers between classes replace kernel boundaries; replicas are independent chains, so no
         workgroup waits for another). Ising: a site whose table value decides it makes no draw; the others draw
         at once, so one barrier per class pass suffices (queueing them for dense draws cost a second barrier and
-        was 20 % slower at the 64^2 x 512 bench shape). With 2 to 4 sites per thread and class, a thread reads all
-        its sites' neighbours before it decides or draws any of them (17 % faster at that shape)."""
+        was 20 % slower at the 64^2 x 512 bench shape). With several sites per thread and class, a thread reads
+        the neighbours of up to RESIDENT_GROUP of them before it decides or draws any (17 % faster at that shape,
+        2-6 % on 192^2 triangular and 32^3 sc lattices at B = 128)."""
         w = self.w
         T = "int8_t" if self.mode == MODE_INT8 else "float"
         RT = self.RESIDENT_THREADS
@@ -1001,14 +1003,18 @@ class _Gen:
         nit = -(-slots // RT)
         for k, (c, a) in enumerate(self.classes):
             w(f"        // class {k}: colour {c}, sublattice {a}")
-            if self.n == 1 and 2 <= nit <= 4:
-                # Ising: all of a thread's sites read their neighbours and table value first (the LDS reads of
-                # every site in flight together), then decide or draw; sites of one class are not neighbours
+            if self.n == 1 and nit >= 2:
+                # Ising: all of a thread's sites (in groups of at most RESIDENT_GROUP) read their neighbours and
+                # table value first (the LDS reads of every site in flight together), then decide or draw;
+                # sites of one class are not neighbours
+                nb = min(nit, RESIDENT_GROUP)
                 w("        {")
-                w(f"            int own_[{nit}], s0_[{nit}]; float p_[{nit}]; bool valid_[{nit}];")
+                if nb < nit:
+                    w(f"        for (int g0 = 0; g0 < {nit}; g0 += {nb}) {{")
+                w(f"            int own_[{nb}], s0_[{nb}]; float p_[{nb}]; bool valid_[{nb}];")
                 w("#pragma unroll")
-                w(f"            for (int u = 0; u < {nit}; ++u) {{")
-                w(f"                const int t = threadIdx.x + u * {RT};")
+                w(f"            for (int u = 0; u < {nb}; ++u) {{")
+                w(f"                const int t = threadIdx.x + ({'g0 + ' if nb < nit else ''}u) * {RT};")
                 w(f"                const int row = t / {self.BX};")
                 w(f"                const int j = t - row * {self.BX};")
                 w(f"                const int z = row / {self.L2}, y = row - (row / {self.L2}) * {self.L2};")
@@ -1025,7 +1031,7 @@ class _Gen:
                 w("                own_[u] = own; valid_[u] = valid;")
                 w("            }")
                 w("#pragma unroll")
-                w(f"            for (int u = 0; u < {nit}; ++u) {{")
+                w(f"            for (int u = 0; u < {nb}; ++u) {{")
                 w("                const int own = own_[u], s0 = s0_[u];")
                 w("                const float p = p_[u];")
                 w(f"                if (valid_[u] && {self.decided_expr('p')}) {{")
@@ -1035,6 +1041,8 @@ class _Gen:
                 w(f"                    if ({self.accept_expr('p')}) lat[own] = ({T})(-s0);")
                 w("                }")
                 w("            }")
+                if nb < nit:
+                    w("        }")
                 w("        }")
                 w("        __syncthreads();")
                 continue

@@ -73,7 +73,7 @@ SMALL = [lambda: sy.ising("square", 4), lambda: sy.xy("square", 4), lambda: sy.h
          lambda: sy.heisenberg("fcc", 2),
          lambda: sy.System("XY sc 10x5x5 periodic", 2, cr.make("sc"), (10, 5, 5), (True,) * 3, {1: 1.0}),
          lambda: sy.nanodot(Rd=2.5, L=3, shells=(1, 2, 4), J_shells={2: 0.3, 4: -0.1}),
-         lambda: sy.heisenberg("triangular", 24), lambda: sy.xy("square", 32)]
+         lambda: sy.heisenberg("triangular", 24), lambda: sy.xy("square", 32), lambda: sy.ising("square", 32)]
 
 
 @pytest.mark.parametrize("i", range(len(SMALL)))
@@ -93,9 +93,10 @@ def test_parallel_equals_sequential_chain(i):
     assert int(A.state[1]) == 2 and int(R.state[1]) == 2
 
 
-@pytest.mark.parametrize("i", [0, 2, 4, 5, 9])
+@pytest.mark.parametrize("i", [0, 2, 4, 5, 9, 12])
 def test_resident_kernel_equals_class_launches(i):
-    """Whole sweeps in LDS (one launch for n sweeps) equal the per-class launches spin for spin."""
+    """Whole sweeps in LDS (one launch for n sweeps) equal the per-class launches spin for spin (Ising square 32
+    with 64 threads: 8 sites per thread and class, read in two groups of 4 before their draws)."""
     s = SMALL[i]()
     print(f"[gpu] resident == class launches {s.name} {s.box}", flush=True)
     C = Sampler(s, 5, DEV, seed=4, resident=False)
