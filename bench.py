@@ -1,16 +1,20 @@
-"""Throughput of the multi-colour sweep on one MI355X: site updates/s and fraction of the HBM roofline.
+"""Throughput of the multi-colour sweep on one MI355X: site updates/s and fraction of the HBM roofline, and
+effective samples/s of the learned proposal width against fixed-width Metropolis.
 
-Default workload (the configuration the headline metric is quoted on): bitwise multi-spin Ising, simple cubic
-L = 128 periodic, J = 1, h = 0, B = 1024 replicas packed 32 per uint32 word with independent per-replica Philox
-draws, T = 4.5, random +-1 start. A step is one coloured sweep (2 class launches + the counter bump) over all
-replicas; inputs are resident in HBM before the timed region.
+Default workload (the largest single-GPU configuration of the headline metric): XY model on the triangular
+lattice, L = 2048 periodic (4 colour classes), J1 = 1, h = 0, fp32 2-vectors with uniform random angles,
+B = 2048 replicas = 16 temperatures x 128 in [0.7, 1.1]. A step is one coloured sweep over all replicas; inputs
+are resident in HBM before the timed region, which replays a HIP graph of exactly `--steps` sweeps.
 
-    python bench.py [--steps K] [--warmup W] [--workload bitwise|ising|heisenberg|xy|learned|nanodisks]
+    python bench.py [--steps K] [--warmup W] [--workload xy|bitwise|ising|heisenberg|nanodisks|learned|...]
 
-Prints one JSON line. `roofline` is for the dominant kernel (the class-update kernel): algorithmic bytes per
-launch (compulsory read + write of each spin word: 0.25 B per site update) over the launch's average duration,
-measured with HIP events on the stream the kernels run on. `cpu_baseline` times the CPU oracle (the same chain,
-sequential, in C) on a bounded sample on this host.
+Prints one JSON line. `roofline` is for the dominant kernel: algorithmic bytes per launch (compulsory read +
+write of each spin, 16 B per XY site update) over the launch's average duration, measured with HIP events on
+the stream the kernels run on; `valu` gives the same kernel's vector-instruction issue fraction from its
+committed counter run. `cpu_baseline` times the CPU oracle (the same chain, sequential, in C) on a bounded
+sample on this host. `ess` (default workload only) is the learned-versus-fixed comparison on the Heisenberg
+bcc 32^3 configuration (B = 8192): effective samples/s of the energy, learned width over the best of five
+fixed widths and over the uniform redraw.
 """
 
 from __future__ import annotations
@@ -112,13 +116,24 @@ def prebuilt_specs():
 
 
 def _cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it (model name, family, model, stepping)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
     except OSError:
         pass
-    return "unknown"
+    name = info.get("model name", "unknown")
+    rest = ", ".join(f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info)
+    return f"{name} ({rest})" if rest else name
+
+
+def _host_cores() -> dict:
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "usable": usable}
 
 
 def cpu_baseline(workload: str, s, A, S0, budget_s: float = 10.0) -> dict:
@@ -155,7 +170,7 @@ def cpu_baseline(workload: str, s, A, S0, budget_s: float = 10.0) -> dict:
     vn, swn, dtn = run(ncores, ncores, budget_s)
     return {"value": v1, "unit": "site updates/s", "cores": 1, "kind": "port",
             "sample": f"1 {unit} x {s.n_active} sites x {sw1} sweeps, sequential chain, {dt1:.1f} s",
-            "cpu_model": _cpu_model(),
+            "cpu_model": _cpu_model(), "host_cores": _host_cores(),
             "all_cores": {"value": vn, "cores": ncores,
                           "sample": f"{ncores} x 1 {unit} (one per thread, OpenMP) x {s.n_active} sites x {swn} "
                                     f"sweeps, {dtn:.1f} s"}}
@@ -214,6 +229,17 @@ def traffic_from_profiles(workload: str):
     f = REPO / "profiles" / f"traffic_{workload}.json"
     if f.exists():
         return json.loads(f.read_text()).get("bytes_per_launch")
+    return None
+
+
+def valu_from_profiles(workload: str):
+    """Vector-instruction issue fraction of the dominant kernel from its committed rocprofv3 counter run
+    (4 x SQ_INSTS_VALU / (1024 SIMDs x cycles per XCD)), if any."""
+    f = REPO / "profiles" / f"valu_{workload}.json"
+    if f.exists():
+        d = json.loads(f.read_text())
+        return {"frac": d.get("valu_issue_frac"), "salu_frac": d.get("salu_issue_frac"),
+                "valu_per_update": d.get("valu_per_update"), "source": f"profiles/valu_{workload}.json"}
     return None
 
 
@@ -422,7 +448,7 @@ def nanodisk_main(a) -> None:
             sw2 += 1
         dt2 = time.perf_counter() - t2
         out["cpu_baseline"] = {"value": systems[med].n_active * sweeps / dt, "unit": "site updates/s", "cores": 1,
-                               "kind": "port", "cpu_model": _cpu_model(),
+                               "kind": "port", "cpu_model": _cpu_model(), "host_cores": _host_cores(),
                                "sample": f"1 replica of the median disk ({systems[med].n_active} sites) x {sweeps} "
                                          f"sweeps, sequential chain, {dt:.1f} s",
                                "all_cores": {"value": ncores * systems[med].n_active * sw2 / dt2, "cores": ncores,
@@ -434,10 +460,11 @@ def nanodisk_main(a) -> None:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="bitwise", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="xy", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ess", action="store_true", help="default workload: skip the learned-vs-fixed comparison")
     ap.add_argument("--tile", default=None, help="replica-major kernels: class slots per tile, e.g. 8,8,4")
     ap.add_argument("--chunks", type=int, default=None, help="replica-major kernels: 64-replica chunks per wave")
     ap.add_argument("--resident-threads", type=int, default=None, help="LDS-resident kernel: threads per replica")
@@ -512,35 +539,26 @@ def main() -> None:
     ncls = len(A.classes)
     upd_per_sweep = B * s.n_active
     resident = A.module.resident is not None and A.use_resident
-    # a resident launch runs `per` whole sweeps; otherwise one launch is one class of one sweep
-    per = 10 if resident else a.steps
-    nl = max(1, a.steps // per)
-    steps = nl * per
-    A.sweep(S, a.warmup)
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    steps = max(1, a.steps)
+    if a.warmup:
+        A.sweep(S, a.warmup)
+    # the timed region: one replay of a HIP graph of exactly `steps` sweeps (class launches and counter bumps)
+    rp = A.graph(S, steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(nl):
-        A.sweep(S, per)
-    e1.record(stream)
-    torch.cuda.synchronize()
+    ms = rp.time_ms(1)
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1)
     sec = ms / 1e3
     value = upd_per_sweep * steps / sec
+    del rp
     # dominant kernel: the class launches (or the resident kernel). Its own duration comes from HIP event
     # pairs around each of its launches on the sampler's stream, over further sweeps right after the timed
     # region (the counter bump and launch gaps stay outside the pairs)
-    launches = nl if resident else steps * len(A.module.sweep)     # class launches (per replica block)
-    bytes_per_launch = upd_per_sweep * steps / launches * alg_bytes
-    kms, kn = 0.0, 0
-    for _ in range(min(nl, 20) if resident else 1):
-        m_, n_ = A.sweep_timed(S, per if resident else min(steps, 50))
-        kms, kn = kms + m_, kn + n_
+    # (a resident launch runs all `steps` sweeps of the timed graph, a class launch one class of one sweep)
+    tsweeps = steps if resident else min(steps, 50)
+    kms, kn = A.sweep_timed(S, tsweeps)
     launch_s = kms / 1e3 / kn
+    bytes_per_launch = upd_per_sweep * tsweeps / kn * alg_bytes
     achieved = bytes_per_launch / launch_s / 1e9
     out = {
         "metric": "site updates/s (%HBM roofline)",
@@ -554,20 +572,53 @@ def main() -> None:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bitwise": "u32", "ising": "int8", "heisenberg": "f32", "xy": "f32"}[a.workload],
-        "data": "synthetic (random initial spins from the seeded torch CPU generator)",
+        "data": "synthetic (random initial spins: seeded torch generator)",
         "config": {"workload": WORKLOADS[a.workload], "batch": B, "sites": s.n_active, "classes": ncls,
                    "kernel": "sck_resident" if resident else "sck_class_*", "layout": A.layout,
-                   "replica_blocks": A.module.rep_blocks,
+                   "replica_blocks": A.module.rep_blocks, "timed": f"one replay of a HIP graph of {steps} sweeps",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload),
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6,
                      "launches_timed": kn},
+        "valu": valu_from_profiles(a.workload),
         "wall_s": wall,
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.workload, s, A, S0)
-    print(json.dumps(out), flush=True)
+    if a.workload == "xy" and not a.no_ess:
+        del A, S
+        torch.cuda.empty_cache()
+        try:
+            out["ess"] = ess_summary()
+        except Exception as e:                 # the throughput line stands on its own
+            out["ess"] = {"error": f"{type(e).__name__}: {e}"}
+    print(json.dumps(out, default=float), flush=True)
+
+
+def ess_summary(log=None) -> dict:
+    """Effective samples/s of the energy on the Heisenberg bcc 32^3 configuration (B = 8192, 32 temperatures in
+    [0.5, 3]): the learned width over the best of five fixed widths and over the uniform redraw, with the
+    per-temperature best fixed width (the ceiling of any width-only proposal) beside them."""
+    import numpy as np
+
+    from sck_amd.ess import compare_proposals
+    s, _, B, T, _ = _make("heisenberg")
+    temps = np.unique(T.numpy())
+    r = compare_proposals(s, B, temps, warmup=200, equil=1000, therm=50, n_meas=400,
+                          log=log or (lambda m: print(m, file=sys.stderr, flush=True)))
+    return {"metric": "effective samples/s, learned vs fixed proposal",
+            "workload": WORKLOADS["heisenberg"],
+            "protocol": "train on 200 fixed-width warm-up sweeps; 1000 learned-width sweeps to one common "
+                        "equilibrium state; from it 50 + 400 sweeps of each proposal with an observation after "
+                        "every sweep; tau_int of E per temperature (Sokal window, c = 6)",
+            "gain_E": r["gain_E"], "gain_E_vs_uniform": r["vs_uniform"]["gain_E"],
+            "gain_E_vs_per_T_best_fixed": r["gain_E_vs_per_T_best"],
+            "per_T_best_fixed_over_best_single_fixed": r["gain_E_per_T_best_over_fixed"],
+            "gain_M": r["gain_M"], "fixed_width": r["fixed_width"],
+            "observables_frac_within_2sigma": r["frac_within_2sigma"], "observables_max_z": r["max_z"],
+            "ess_E_per_s": {k: r[k]["ess_E_per_s"] for k in ("uniform", "fixed", "learned")},
+            "train_seconds": r["train_seconds"]}
 
 
 if __name__ == "__main__":

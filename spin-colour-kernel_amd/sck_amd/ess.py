@@ -125,6 +125,16 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     result["fixed_scan_gm_ess"] = {k: gm(result[k]) for k in fixed_labels + ["uniform", "learned"]}
     log(f"  best fixed width by measured ESS/s: {result['fixed_width']}")
     result.update(_gains(result["fixed"], result["learned"], nt))
+    # the ceiling of any width-only proposal that is constant per temperature: at every temperature the best
+    # of the scanned widths and the uniform redraw, each by its own measured ESS/s of E
+    cands = fixed_labels + ["uniform"]
+    best_T = [max(cands, key=lambda k: 1.0 / (result[k]["tau_E"][j] * result[k]["seconds"])) for j in range(nt)]
+    ceil = np.array([1.0 / (result[k]["tau_E"][j] * result[k]["seconds"]) for j, k in enumerate(best_T)])
+    lrn = np.array([1.0 / (result["learned"]["tau_E"][j] * result["learned"]["seconds"]) for j in range(nt)])
+    result["per_T_best_fixed"] = best_T
+    result["gain_E_per_T_best_over_fixed"] = float(np.exp(np.mean(np.log(
+        ceil * np.array([result["fixed"]["tau_E"][j] * result["fixed"]["seconds"] for j in range(nt)])))))
+    result["gain_E_vs_per_T_best"] = float(np.exp(np.mean(np.log(lrn / ceil))))
     result["vs_uniform"] = _gains(result["uniform"], result["learned"], nt)
     result["fixed_vs_uniform"] = _gains(result["uniform"], result["fixed"], nt)
     # agreement of the observables: learned against each baseline

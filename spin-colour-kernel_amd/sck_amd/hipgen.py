@@ -24,12 +24,17 @@ the sequential kernel, the class kernels and the CPU oracle agree:
     built on the host with numpy float32 exp, exactly the values the reference's CPU chain computes. The
     draw is skipped where the table decides alone (p >= 1 or p == 0): the Philox stream is counter-based, so
     skipping a draw changes no other draw.
-  * XY / Heisenberg: fp32 local field, proposal and dE in the reference's order; cos/sin/exp are the device
-    library's accurate versions (parity: decision tolerance).
+  * XY / Heisenberg: fp32 local field, proposal and dE in the reference's order. cos and sin come from
+    sincos_bounded (Cody-Waite reduction and the Cephes single-precision polynomials, error a few ulp for the
+    bounded arguments used here) and exp from the hardware base-2 exponential of x log2(e) (exp_fast: relative
+    error about |x| 2^-24 plus an ulp, under 3e-6 wherever |beta dE| <= 50). Neither is the reference's numpy
+    float32 function, so parity is the decision tolerance: equal decisions except where |beta dE + ln u| < 1e-5.
 
-Philox-4x32-10: key (seed, 0), counter (rep * nsite + own) split into (low, high) words -- for counters below
-2^31 exactly the reference's int32 counter (SCK/generator.py:275-276), above it the high word Triton's 64-bit
-offsets would use (triton/language/random.py:102-110).
+Philox-4x32-10: key (seed, 0), counter (rep * nsite + own). Below 2^31 this is exactly the reference's int32
+counter (SCK/generator.py:275-276). The reference refuses every batch with Bp * nsite >= 2^31 - 1
+(SCK/sampler.py:73-74); this build instead carries the counter in 64 bits, low word in c0 and high word in c1,
+so larger batches (bitwise Ising sc 128^3 x 1024, XY triangular 2048^2 x 2048) run. That is a deliberate
+extension outside the reference's range, pinned by the oracle's restatement of the same rule.
 """
 
 from __future__ import annotations
@@ -40,7 +45,7 @@ from dataclasses import dataclass, field, replace
 
 import numpy as np
 
-from .system import System
+from .system import System, as_system
 
 __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT8", "MODE_F32", "MODE_PACKED",
            "MODE_BITWISE", "MODE_BITWISE_INDEP", "BLOCK", "BatchModule", "generate_batch", "batch_stride",
@@ -1985,12 +1990,12 @@ def generate_batch(systems, mode: int, proposal: str = "uniform", learned_meta=N
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and systems[0].n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _BatchGen(list(systems), mode, proposal, learned_meta, wide).generate_batch()
+    return _BatchGen([as_system(x) for x in systems], mode, proposal, learned_meta, wide).generate_batch()
 
 
 def fits_resident(sys_: System, mode: int) -> bool:
     """Whether one replica's lattice fits the LDS-resident kernel."""
-    return _Gen(sys_, mode, 1, "uniform").resident_plan() is not None
+    return _Gen(as_system(sys_), mode, 1, "uniform").resident_plan() is not None
 
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
@@ -2002,5 +2007,5 @@ def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", lea
         raise ValueError("proposal must be 'uniform', 'cone' or 'learned'")
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
-    return _Gen(sys_, mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites,
+    return _Gen(as_system(sys_), mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites,
                 resident_threads, rm_block).generate()

@@ -131,6 +131,24 @@ class System:
                 if np.any((cm[a] == c) & m[a])]
 
 
+def as_system(obj) -> System:
+    """`obj` as a System. An object that only carries the reference System's fields (SCK/system.py:19-34: name,
+    n, crystal, box, periodic, J, extra, h, cubic_k1, cubic_k2, mask, colouring, T_ref, acceptance), such as
+    the reference's own System, gets the terms it lacks as None (no uniaxial or per-site anisotropy); its
+    crystal and colouring are used as they are (their bond table and (m, K, o) fix the kernel), and the class
+    list is derived from its colours and mask as SCK/sampler.py:127-129 does."""
+    if isinstance(obj, System):
+        return obj
+    for f in ("name", "n", "crystal", "box", "periodic", "J", "colouring"):
+        if not hasattr(obj, f):
+            raise ValueError(f"not a spin system: no field {f!r}")
+    return System(str(obj.name), int(obj.n), obj.crystal, tuple(obj.box), tuple(obj.periodic), dict(obj.J),
+                  dict(getattr(obj, "extra", None) or {}), getattr(obj, "h", None), getattr(obj, "cubic_k1", None),
+                  float(getattr(obj, "cubic_k2", 0.0) or 0.0), getattr(obj, "mask", None), obj.colouring,
+                  getattr(obj, "T_ref", None), getattr(obj, "acceptance", "metropolis"),
+                  getattr(obj, "uniaxial", None), getattr(obj, "site_ku", None), getattr(obj, "site_axis", None))
+
+
 # -------------------------------------------------------------------- matrix
 def ising(crystal: str, L: int, J: float = 1.0, shells=(1,), T_ref=None) -> System:
     c = cr.make(crystal, tuple(shells))
