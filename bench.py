@@ -38,7 +38,7 @@ WORKLOADS = {
     "bitwise": "bitwise Ising sc L=128 periodic, B=1024 (32 words), independent draws, T=4.5",
     "ising": "Ising square L=64 periodic, int8, B=512 (8 temps x 64 in [1.5,3.5]), seed 0",
     "heisenberg": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, fp32, B=8192 (32 temps x 256 in [0.5,3.0])",
-    "xy": "XY triangular L=2048 periodic J1=1, fp32, B=2048 (16 temps x 128 in [0.7,1.1]), replica-major",
+    "xy": "XY triangular L=2048 periodic J1=1, fp32, B=2048 (16 temps x 128 in [0.7,1.1])",
     "learned": "Heisenberg bcc L=32 J1=1 J2=-0.2 Du=0.1 z h=0.05 z, B=8192 (32 temps in [0.5,3.0]): effective "
                "samples/s of the learned cone width over the best of five fixed widths (by measured ESS/s)",
     "nanodisks": "64 Heisenberg sc nanodisks, radius U[8,40], thickness U{4..12} (numpy default_rng(2024)), open, "
@@ -99,19 +99,20 @@ def _make(workload: str):
 
 
 def prebuilt_specs():
-    """(system, mode, batch, replica_major) of every bench workload, for the build step's kernel
+    """(system, mode, batch, replica_major, tiled) of every bench workload, for the build step's kernel
     pre-compilation (the layout is the one the Sampler picks for it)."""
     from sck_amd import hipgen as G
+    from sck_amd.sampler import RESIDENT_MIN_BATCH, RESIDENT_SMALL_SITES, choose_layout
     out = []
     for wl in ("bitwise", "ising", "heisenberg", "xy"):
         s, kw, B, _, _ = _make(wl)
         if kw.get("bitwise"):
-            out.append((s, G.MODE_BITWISE_INDEP, B, False))
-        elif s.n == 1:
-            out.append((s, G.MODE_INT8, B, False))
-        else:
-            rm = B % 64 == 0 and not G.fits_resident(s, G.MODE_F32)
-            out.append((s, G.MODE_F32, B, rm))
+            out.append((s, G.MODE_BITWISE_INDEP, B, False, False))
+            continue
+        mode = G.MODE_INT8 if s.n == 1 else G.MODE_F32
+        res = B >= RESIDENT_MIN_BATCH or s.nsite <= RESIDENT_SMALL_SITES
+        layout, tiled = choose_layout(s, mode, B, res)
+        out.append((s, mode, B, layout == "replica", tiled))
     return out
 
 
@@ -468,6 +469,9 @@ def main() -> None:
     ap.add_argument("--tile", default=None, help="replica-major kernels: class slots per tile, e.g. 8,8,4")
     ap.add_argument("--chunks", type=int, default=None, help="replica-major kernels: 64-replica chunks per wave")
     ap.add_argument("--resident-threads", type=int, default=None, help="LDS-resident kernel: threads per replica")
+    ap.add_argument("--tile-dims", default=None, help="tiled sweep kernel: cells per tile along x and y, e.g. 64,64")
+    ap.add_argument("--layout", default="auto", choices=("auto", "reference", "replica"),
+                    help="spin storage (replica: lanes over replicas, per-class launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference for the HBM counters)")
     ap.add_argument("--calibrate-cache-copy", action="store_true",
@@ -529,6 +533,11 @@ def main() -> None:
         kw["chunks"] = a.chunks
     if a.resident_threads:
         kw["resident_threads"] = a.resident_threads
+    if a.tile_dims:
+        kw["tile_dims"] = tuple(int(v) for v in a.tile_dims.split(","))
+    if a.layout != "auto":
+        kw["layout"] = a.layout
+        kw["tiled"] = False if a.layout == "replica" else None
     A = Sampler(s, B, "cuda:0", seed=0, **kw)
     A.set_temperature(T)
     if a.workload in ("xy", "heisenberg"):
@@ -574,7 +583,8 @@ def main() -> None:
         "dtype": {"bitwise": "u32", "ising": "int8", "heisenberg": "f32", "xy": "f32"}[a.workload],
         "data": "synthetic (random initial spins: seeded torch generator)",
         "config": {"workload": WORKLOADS[a.workload], "batch": B, "sites": s.n_active, "classes": ncls,
-                   "kernel": "sck_resident" if resident else "sck_class_*", "layout": A.layout,
+                   "kernel": "sck_resident" if resident else ("sck_tiled" if A.use_tiled else "sck_class_*"),
+                   "layout": A.layout, "tile_dims": A.module.tile_dims if A.use_tiled else None,
                    "replica_blocks": A.module.rep_blocks, "timed": f"one replay of a HIP graph of {steps} sweeps",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SCK_ABI_VERSION 3
+#define SCK_ABI_VERSION 4
 
 enum sck_status {
     SCK_OK = 0,
@@ -96,6 +96,11 @@ typedef struct sck_desc {
                                  all n sweeps before the next (replicas are independent chains, so the chain
                                  of every replica is unchanged) and a block's lattice stays cache-resident */
     int32_t pad1;
+    const sck_launch *tiled;                     /* NULL, or one whole sweep in one launch: each workgroup runs
+                                                    every class of one (replica, tile) in LDS, reading
+                                                    args.spins and writing args.dst. Whole sweeps use it when
+                                                    args.dst is a workspace of the spins' size (unless
+                                                    `resident` is set); otherwise the class launches run */
 } sck_desc;
 
 /* ---------------------------------------------------------------- learned proposal width (no reference
@@ -152,7 +157,9 @@ int sck_precompile(const char *hip_src, const char *cache_dir);
 int sck_destroy(sck_handle *h);
 
 /* n coloured sweeps in place, then state[1] += n (SCK/sampler.py:234-240). stream: a hipStream_t or NULL.
- * class_lo/class_hi select launches [lo, hi) of one sweep (0, -1: all); used by per-class checks. */
+ * class_lo/class_hi select launches [lo, hi) of one sweep (0, -1: all); used by per-class checks.
+ * A module with a tiled launch runs whole sweeps through it when args->dst points to a workspace of the spins'
+ * size: the sweeps alternate between the two buffers and the result ends in args->spins. */
 int sck_sweep(sck_handle *h, const sck_args *args, int32_t n, int32_t class_lo, int32_t class_hi, void *stream);
 
 /* sck_sweep of n whole sweeps with a HIP event pair on `stream` around every sweep-kernel launch (the class

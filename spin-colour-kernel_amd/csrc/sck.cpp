@@ -150,6 +150,9 @@ struct sck_handle {
     sck_launch bump{};
     bool has_resident = false;
     sck_launch resident{};
+    bool has_tiled = false;
+    sck_launch tiled{};
+    size_t spin_bytes = 0;    // float32 spins of the whole batch (the tiled sweep's buffers)
     int sample_kernel = -1;
     int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
     int rep_blocks = 1;         // the sweep list holds rep_blocks groups of class launches, one per replica block
@@ -220,6 +223,20 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
         a.nsweep = n;
         int rc = timed_launch(h, h->resident, a, s, t);
         if (rc) return rc;
+        return bump(h, *args, n, s);
+    }
+    if (h->has_tiled && lo == 0 && hi == ns && nb == 1 && args->dst && n > 0) {
+        // whole sweeps: one launch per sweep, every class of a (replica, tile) in LDS. A tile reads its halo
+        // from other tiles' cells, so a sweep reads one buffer and writes the other (args->dst, a workspace of
+        // the spins' size); after an odd number of sweeps the result is copied back into args->spins
+        for (int32_t i = 0; i < n; ++i) {
+            a.sweep_offset = i;
+            a.spins = (i & 1) ? args->dst : args->spins;
+            a.dst = (i & 1) ? args->spins : args->dst;
+            int rc = timed_launch(h, h->tiled, a, s, t);
+            if (rc) return rc;
+        }
+        if (n & 1) HIP_TRY(hipMemcpyAsync(args->spins, args->dst, h->spin_bytes, hipMemcpyDeviceToDevice, s));
         return bump(h, *args, n, s);
     }
     // each replica block runs all n sweeps before the next one starts
@@ -295,6 +312,35 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
     if (d->resident) {
         h->has_resident = true;
         h->resident = *d->resident;
+    }
+    if (d->tiled) {
+        if (d->mode != SCK_MODE_F32) {
+            sck_destroy(h);
+            return fail(SCK_E_INVALID, "the tiled sweep runs float32 spins");
+        }
+        h->has_tiled = true;
+        h->tiled = *d->tiled;
+        if (h->tiled.kernel < 0 || h->tiled.kernel >= (int)h->fn.size()) {
+            sck_destroy(h);
+            return fail(SCK_E_INVALID, "tiled launch names an unknown kernel");
+        }
+        if (h->tiled.grid[1] == 0) {
+            // persistent workgroups: as many per XCD column (grid.x) as are resident at once on the device
+            int dev = 0, cus = 0, per = 0;
+            hipError_t e1 = hipGetDevice(&dev);
+            if (e1 == hipSuccess) e1 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e1 == hipSuccess)
+                e1 = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per, h->fn[h->tiled.kernel],
+                                                                       (int)h->tiled.block[0], h->tiled.shmem);
+            if (e1 != hipSuccess) {
+                sck_destroy(h);
+                return fail(SCK_E_HIP, std::string("tiled occupancy: ") + hipGetErrorString(e1));
+            }
+            const unsigned cols = h->tiled.grid[0] ? h->tiled.grid[0] : 1;
+            const unsigned wg = (unsigned)(cus > 0 ? cus : 1) * (unsigned)(per > 0 ? per : 1) / cols;
+            h->tiled.grid[1] = wg > 0 ? wg : 1;
+        }
+        h->spin_bytes = (size_t)d->batch * d->n * d->b * d->L1 * d->L2 * d->L3 * sizeof(float);
     }
     h->mode = d->mode;
     h->n = d->n;

@@ -62,6 +62,10 @@ BW_QUEUE_DIV = 4              # independent bitwise mode: the tail queue holds 1
 BW_WORDS = 2                  # independent bitwise mode: words per thread, sharing the site's index work
 LEVELS = 32
 RM_BLOCK_BYTES = 224 << 20    # replica-major: largest lattice share run through whole sweeps at a time (cache-resident)
+# tiled sweep: tile sizes (cells along x, y) tried in order, the first whose tile and halo fit the LDS budget.
+# XY triangular 2048^2 x 2048: 128 x 48 gives 1.58e11 site updates/s, 128 x 32 1.54e11, 64 x 64 1.52e11,
+# 96 x 64 1.50e11, 64 x 32 1.29e11 (one box, one run each)
+TILE_CANDIDATES = ((128, 48), (64, 64), (64, 48), (64, 32), (32, 32), (24, 24))
 
 
 @dataclass
@@ -90,6 +94,8 @@ class Module:
     ising_layout: list = field(default_factory=list)
     key: str = ""
     resident: Launch | None = None     # whole sweeps of one replica per workgroup with the lattice in LDS
+    tiled: Launch | None = None        # one whole sweep per launch, (replica, tile) per workgroup, tiles in LDS
+    tile_dims: tuple | None = None     # the tiled kernel's cells per tile (x, y)
     rep_blocks: int = 1                # the sweep list is this many groups of class launches, one per replica block
 
 
@@ -97,6 +103,11 @@ def _hexf(v: float) -> str:
     """A constant as the fp32 value the reference bakes into its source (SCK/generator.py:44-45), written as an
     exact C99 hexadecimal literal."""
     return f"{float(np.float32(v)).hex()}f"
+
+
+def a_expr(var: str, stride: int) -> str:
+    """var * stride, folded to 0 for a single sublattice."""
+    return f"{var} * {stride}"
 
 
 def _f64(x: float) -> str:
@@ -108,6 +119,15 @@ def x_period(col) -> int:
     P = 1
     for r, mr in enumerate(col.m):
         P = math.lcm(P, mr // math.gcd(int(col.K[r][0]) % mr or mr, mr))
+    return P
+
+
+def y_period(col) -> int:
+    """Period along y of a linear colouring: lcm over factors of m_r / gcd(K_r2, m_r)."""
+    P = 1
+    for r, mr in enumerate(col.m):
+        ky = int(col.K[r][1]) if len(col.K[r]) > 1 else 0
+        P = math.lcm(P, mr // math.gcd(ky % mr or mr, mr))
     return P
 
 
@@ -317,8 +337,11 @@ DEV int wrap_add(int v, int d, int L) {
 
 class _Gen:
     def __init__(self, sys_: System, mode: int, B: int, proposal: str, learned_meta=None, replica_major=False,
-                 rm_tile=None, rm_chunks=None, rm_sites=None, resident_threads=None, rm_block=None):
+                 rm_tile=None, rm_chunks=None, rm_sites=None, resident_threads=None, rm_block=None, tiled=False,
+                 tile_dims=None):
         self.s = sys_
+        self.want_tiled = bool(tiled)
+        self.tile_dims = None if tile_dims is None else tuple(int(v) for v in tile_dims)
         self.mode = mode
         self.B = int(B)
         self.proposal = proposal
@@ -580,8 +603,18 @@ class _Gen:
         for k in range(n):
             w(f"    float s{k} = (float){self.at('src', k, 'own')};")
         self.emit_local_field(a)
+        self.emit_vector_tail(lambda k, v: self.store('dst', k, 'own', v))
+        w("    return acc;")
+        w("}")
+
+    def emit_vector_tail(self, store, beta: str = "A.beta[rep]", width: str = "A.width[rep]") -> None:
+        """Body of a vector-model update after the local field v<k> of own spin s<k>: draw, proposal t<k>,
+        dE in the reference's fp32 order (SCK/generator.py:274-309), the anisotropy terms, and the Metropolis
+        or Barker decision `acc`; store(k, value) gives the statement writing component k of the accepted
+        trial spin."""
+        s, n, w = self.s, self.n, self.w
         self.emit_draw("    ")
-        self.emit_proposal()
+        self.emit_proposal(width, beta)
         dsum = f"(t0 - s0) * v0"
         for k in range(1, n):
             dsum = f"({dsum}) + (t{k} - s{k}) * v{k}"
@@ -595,12 +628,9 @@ class _Gen:
             w("      dE = dE + (a_t - a_s); }")
         if s.uniaxial is not None:
             Du, e = s.uniaxial
-            se = " + ".join(f"s{k} * {_hexf(e[k])}" for k in range(n))
-            te = " + ".join(f"t{k} * {_hexf(e[k])}" for k in range(n))
             w(f"    {{ const float se = {self._lassoc([f's{k} * {_hexf(e[k])}' for k in range(n)])};"
               f" const float te = {self._lassoc([f't{k} * {_hexf(e[k])}' for k in range(n)])};")
             w(f"      dE = dE + {_hexf(Du)} * (se * se - te * te); }}")
-            del se, te
         if s.site_ku is not None:
             # only sites with a surface term read its axis: where ku == 0 the term adds a signed zero, which
             # leaves exp(-beta dE) (and Barker's 1 + exp(beta dE)) unchanged
@@ -608,22 +638,22 @@ class _Gen:
             w(f"      const float sn = {self._lassoc([f's{k} * ax[{k}]' for k in range(n)])};"
               f" const float tn = {self._lassoc([f't{k} * ax[{k}]' for k in range(n)])};")
             w("      dE = dE + ku * (sn * sn - tn * tn); } }")
-        w("    const float beta = A.beta[rep];")
+        w(f"    const float beta = {beta};")
         if s.acceptance == "barker":
             w("    const int acc = (u0 * (1.0f + exp_fast(beta * dE))) < 1.0f;")
         else:
             w("    const int acc = u0 < exp_fast((-beta) * dE);")
         w("    if (acc) {")
         for k in range(n):
-            w(f"        {self.store('dst', k, 'own', f't{k}')}")
+            w(f"        {store(k, f't{k}')}")
         w("    }")
-        w("    return acc;")
-        w("}")
 
-    def emit_local_field(self, a: int, ind: str = "    ") -> None:
+    def emit_local_field(self, a: int, ind: str = "    ", nb=None) -> None:
         """v0..v{n-1}: the local field of site (x, y, z) of sublattice a read from `src` at replica offset
         `rbase`, in the reference's fp32 order: J groups in first-appearance order (v = v + J g), then the
-        extra matrices of the extra bonds in bond order, then the field (SCK/generator.py:222-277)."""
+        extra matrices of the extra bonds in bond order, then the field (SCK/generator.py:222-277).
+        nb(i, k), when given, is the expression of component k of bond i's neighbour (always present: a tile in
+        LDS holds zeros outside an open box)."""
         s, n, w = self.s, self.n, self.w
         groups, extras = _groups(s, a)
         for k in range(n):
@@ -636,6 +666,14 @@ class _Gen:
             for k in range(n):
                 w(f"{ind}    float g{k} = 0.0f;")
             for i in members:
+                if nb is not None:
+                    w(f"{ind}    {{")
+                    for k in range(n):
+                        w(f"{ind}      const float t{k} = {nb(i, k)}; g{k} = g{k} + t{k};")
+                        if i in extras:
+                            w(f"{ind}      n{extras.index(i)}_{k} = t{k};")
+                    w(f"{ind}    }}")
+                    continue
                 q, ok = self.nb_index(i)
                 if ok == "true":
                     w(f"{ind}    {{ const int q = {q};")
@@ -692,8 +730,9 @@ class _Gen:
             w(f"{ind}const float u1 = u01(c1);")
             w(f"{ind}const float u2 = u01(c2);")
 
-    def emit_proposal(self) -> None:
-        """Trial spin t from (u1, u2): uniform redraw (SCK/generator.py:281-289) or a cone of half-angle w."""
+    def emit_proposal(self, width: str = "A.width[rep]", beta: str = "A.beta[rep]") -> None:
+        """Trial spin t from (u1, u2): uniform redraw (SCK/generator.py:281-289) or a cone of half-angle w
+        (`width`: the fixed cone's half-angle of this replica, `beta`: its inverse temperature)."""
         w, n = self.w, self.n
         TWO_PI = "6.283185307179586f"
         if self.proposal == "uniform":
@@ -709,13 +748,13 @@ class _Gen:
             return
         # cone proposal
         if self.proposal == "cone":
-            w("    const float wd = A.width[rep];")
+            w(f"    const float wd = {width};")
         else:
             # learned width: the frozen network tabulated on quantised features of the site's environment
             lm = self.lm
             vv = self._lassoc([f"v{k} * v{k}" for k in range(n)])
             w(f"    const float hv = sqrtf({vv});")
-            w("    const float bt = A.beta[rep];")
+            w(f"    const float bt = {beta};")
             kexpr = _hexf(abs(self.s.uniaxial[0]) if self.s.uniaxial is not None else 0.0)
             if self.s.site_ku is not None:
                 kexpr = f"({kexpr} + fabsf(A.site_ku[own]))"
@@ -954,6 +993,234 @@ class _Gen:
         for q in range(self.P - 1, -1, -1):
             w(f"    if ({self.colour_expr(str(q), a)} == {c}) x0 = {q};")
         w(f"    if (x0 >= {self.P}) return;")
+
+    # ------------------------------------------------------------------------------- tiled sweep kernel
+    TILE_THREADS = 256
+    TILE_LDS_MAX = 64 * 1024
+
+    def tiled_plan(self):
+        """(TX, TY, HX, HY) of the tiled sweep kernel, or None where it does not apply: float32 vector spins on a
+        2-D lattice in the reference layout, with a tile and its halo fitting the LDS budget. The halo is the
+        number of classes times the bond reach along each axis: class k of K is updated on the tile grown by
+        (K - 1 - k) reaches, so that the K-th class pass is exact on the tile itself."""
+        if self.mode != MODE_F32 or self.n < 2 or self.D != 2 or self.replica_major or not self.classes:
+            return None
+        rx = max((abs(bd.delta[0]) for bd in self.s.crystal.bonds), default=0)
+        ry = max((abs(bd.delta[1]) for bd in self.s.crystal.bonds), default=0)
+        K = len(self.classes)
+        HX, HY = K * rx, K * ry
+        vec = self.L1 % 4 == 0
+        if vec:
+            HX = -(-HX // 4) * 4              # whole float4 columns of halo (the extra cells are loaded only)
+        # tile origins at multiples of the colouring's periods: every tile then holds the same class pattern,
+        # so each class pass enumerates its sites from compile-time row tables
+        px = math.lcm(self.P, 4 if vec else 1)
+        py = y_period(self.s.colouring)
+        for dims in ([self.tile_dims] if self.tile_dims is not None else TILE_CANDIDATES):
+            TX, TY = min(dims[0], self.L1), min(dims[1], self.L2)
+            TX, TY = TX // px * px, TY // py * py
+            if TX == 0 or TY == 0:
+                continue
+            if self.b * (TX + 2 * HX) * (TY + 2 * HY) * self.n * 4 > self.TILE_LDS_MAX:
+                continue
+            if TX + 2 * HX > self.L1 or TY + 2 * HY > self.L2:
+                continue                      # the grown tile would wrap onto itself: small lattices run per class
+            return TX, TY, HX, HY, rx, ry
+        return None
+
+    def emit_tiled(self, plan) -> None:
+        """sck_tiled: whole coloured sweeps, one launch per sweep. A workgroup takes (replica, tile) items: it loads
+        the tile and a halo of K bond reaches into LDS, runs the K class passes in sweep order, class k on the tile
+        grown by K - 1 - k reaches, with a barrier between passes, and writes the tile back. Every draw depends only
+        on (seed, sweep, replica, site), so a halo site updated here and in the neighbouring tile takes the same
+        value in both, and each tile's sites end exactly as after the K class launches. The lattice is read about
+        (1 + 2 K r / T)^2 times and written once per sweep, against K reads and one write for per-class launches.
+
+        A tile's halo cells belong to other tiles, which may finish first, so the sweep reads A.spins and writes
+        A.dst (the launcher alternates the two buffers).
+
+        Workgroups are persistent: gridDim.x = 8 is the XCD, gridDim.y the workgroups resident per XCD (the
+        launcher sets it from the occupancy). The items of XCD x are one contiguous range, tiles of a replica in
+        raster order, which its workgroups stride through, so a tile's halo rows were read shortly before by the
+        tile above it and come from that XCD's L2. Each workgroup loads its next tile into registers while it
+        runs the class passes of the current one. Where the box and halo are multiples of 4 cells along x, tiles
+        move as float4 rows."""
+        w, s, n, b = self.w, self.s, self.n, self.b
+        TX, TY, HX, HY, rx, ry = plan
+        W, H = TX + 2 * HX, TY + 2 * HY
+        HW = H * W
+        NTX, NTY = -(-self.L1 // TX), -(-self.L2 // TY)
+        NT = NTX * NTY
+        RT = self.TILE_THREADS
+        K = len(self.classes)
+        col = s.colouring
+        V = 4 if (self.L1 % 4 == 0 and HX % 4 == 0 and TX % 4 == 0) else 1
+        NCH = W // V                        # V-wide chunks per halo row (W is a multiple of V)
+        NLD = b * H * NCH                   # chunks of one tile and its halo
+        ULD = -(-NLD // RT)                 # chunks per thread
+        NCI = TX // V if V > 1 else TX
+        NST = b * TY * NCI                  # chunks of the tile itself
+        VT = "float4" if V == 4 else "float"
+        NC = b * HW                         # LDS cells per component: the tile is component-planar
+        for a in range(b):
+            offs = {}
+            for i, bd in enumerate(s.crystal.bonds):
+                if bd.alpha == a:
+                    offs[i] = (bd.alpha2 - a) * HW + bd.delta[1] * W + bd.delta[0]
+            # beta and the cone width arrive per item (loaded once, ahead of the next tile's prefetch)
+            w(f"DEV int tupd_{a}(const sck_args &A, float *__restrict__ ld, int li, int rep, int own, uint32_t key, "
+              f"float beta_r, float width_r) {{")
+            for k in range(n):
+                w(f"    float s{k} = ld[{k * NC} + li];")
+            self.emit_local_field(a, nb=lambda i, k, offs=offs: f"ld[{k * NC} + li + ({offs[i]})]")
+            self.emit_vector_tail(lambda k, v: f"ld[{k * NC} + li] = {v};", beta="beta_r", width="width_r")
+            w("    return acc;")
+            w("}")
+            w()
+        items = self.B * NT
+        chunk = -(-items // XCDS)
+        self.tiled_chunk = chunk
+
+        def ax(var_u, L, periodic, out):
+            # wrapped coordinate `out` and validity of unwrapped `var_u`
+            if periodic:
+                if L & (L - 1) == 0:
+                    return f"const int {out} = {var_u} & {L - 1};", "true"
+                return f"const int {out} = {var_u} < 0 ? {var_u} + {L} : ({var_u} >= {L} ? {var_u} - {L} : {var_u});", "true"
+            return f"const int {out} = {var_u};", f"({var_u} >= 0 && {var_u} < {L})"
+
+        cx, vx = ax("gxu", self.L1, self.per[0], "gx")
+        cy, vy = ax("gyu", self.L2, self.per[1], "gy")
+        zero = "make_float4(0.0f, 0.0f, 0.0f, 0.0f)" if V == 4 else "0.0f"
+        w(f'extern "C" __global__ void __launch_bounds__({RT}) sck_tiled(sck_args A) {{')
+        w(f"    __shared__ __attribute__((aligned(16))) float ld[{b * HW * n}];")
+        w(f"    const int64_t lo = (int64_t)blockIdx.x * {chunk};")
+        w(f"    const int64_t hi = lo + {chunk} < {items}LL ? lo + {chunk} : {items}LL;")
+        w("    const int64_t step = gridDim.y;")
+        w("    int64_t g = lo + blockIdx.y;")
+        w("    if (g >= hi) return;")
+        w("    const uint32_t key = sweep_key(A);")
+        # one named register (vector) per prefetched chunk and component, fully unrolled here: an indexed array
+        # would be placed in scratch memory
+        w(f"    {VT} {', '.join(f'p{u}_{k}' for u in range(ULD) for k in range(n))};")
+        # the loads of item gi (issued, not waited for)
+        w("    auto fetch = [&](int64_t gi) {")
+        w(f"        const int rep = (int)(gi / {NT}), tile = (int)(gi - (int64_t)rep * {NT});")
+        w(f"        const int tx0 = (tile % {NTX}) * {TX}, ty0 = (tile / {NTX}) * {TY};")
+        w(f"        const float *S = (const float *)A.spins + (int64_t)rep * {n * self.NS};")
+        for u in range(ULD):
+            w("        {")
+            w(f"            const int i = threadIdx.x + {u * RT};")
+            w(f"            const int a = i / {H * NCH}, r = i - a * {H * NCH};")
+            w(f"            const int ly = r / {NCH}, c = r - ly * {NCH};")
+            w(f"            const int gxu = tx0 - {HX} + c * {V}, gyu = ty0 - {HY} + ly;")
+            w(f"            {cx} {cy}")
+            w(f"            const bool ok = {'i < ' + str(NLD) + ' && ' if (u + 1) * RT > NLD else ''}{vx} && {vy};")
+            # an always-valid address, so the loads issue without branches; cells outside the box (and the
+            # padding chunks) are zeroed or skipped when written to LDS, so nothing waits for these loads before then
+            w(f"            const int64_t q = ok ? (int64_t)a * {self.PL} + (int64_t)gy * {self.L1} + gx : 0;")
+            for k in range(n):
+                w(f"            p{u}_{k} = *(const {VT} *)(S + {self.NS * k} + q);")
+            w("        }")
+        w("    };")
+        w("    fetch(g);")
+        w("    for (; g < hi; g += step) {")
+        w(f"        const int rep = (int)(g / {NT}), tile = (int)(g - (int64_t)rep * {NT});")
+        w(f"        const int tx0 = (tile % {NTX}) * {TX}, ty0 = (tile / {NTX}) * {TY};")
+        # a replica's beta and cone width, made wave-uniform here: the wait for the load falls before the next
+        # tile's prefetch is issued, and the class passes hold them in scalar registers
+        w("        const float beta_r = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane("
+          "__builtin_bit_cast(int, A.beta[rep])));")
+        if self.proposal == "cone":
+            w("        const float width_r = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane("
+              "__builtin_bit_cast(int, A.width[rep])));")
+        else:
+            w("        const float width_r = 0.0f;")
+        w("        __syncthreads();                          // the previous item's write-back has read the LDS")
+        conds = [v for v in (vx, vy) if v != "true"]
+        for u in range(ULD):
+            w("        {")
+            w(f"            const int i = threadIdx.x + {u * RT};")
+            guard = (u + 1) * RT > NLD
+            if guard:
+                w(f"            if (i < {NLD}) {{")
+            w(f"            const int a = i / {H * NCH}, r = i - a * {H * NCH};")
+            w(f"            const int ly = r / {NCH}, c = r - ly * {NCH};")
+            w(f"            float *dl = ld + {a_expr('a', HW)} + ly * {W} + c * {V};")
+            if conds:
+                # outside an open box the LDS holds zeros (no spin there)
+                w(f"            const int gxu = tx0 - {HX} + c * {V}, gyu = ty0 - {HY} + ly;")
+                w(f"            const bool ok = {' && '.join(conds)};")
+            for k in range(n):
+                val = f"(ok ? p{u}_{k} : {zero})" if conds else f"p{u}_{k}"
+                w(f"            *({VT} *)(dl + {k * NC}) = {val};")
+            if guard:
+                w("            }")
+            w("        }")
+        w("        __syncthreads();")
+        w("        if (g + step < hi) fetch(g + step);          // in flight during the class passes")
+        P, Py = self.P, y_period(col)
+        self.tiled_tables = []
+        for k, (c, a) in enumerate(self.classes):
+            e = K - 1 - k
+            x_lo, y_lo = HX - e * rx, HY - e * ry
+            wx, hy = TX + 2 * e * rx, TY + 2 * e * ry
+            # rows of the pass that hold class sites, by row residue mod Py, and their first x residue: the tile
+            # origin (tx0, ty0) is a multiple of (P, Py), so these depend on the local coordinates only
+            xb = (x_lo - HX) % P
+            res = []
+            for r in range(Py):
+                yy = (y_lo - HY + r) % Py
+                q0 = next((q for q in range(P) if col.colour(((xb + q) % P, yy)[:self.D], a) == c), None)
+                if q0 is not None:
+                    res.append((r, q0))
+            self.tiled_tables.append((x_lo, y_lo, wx, hy, res))
+            if not res:
+                continue
+            nv = len(res)
+            bxk = max(-(-(wx - q0) // P) for _, q0 in res)
+            nblk = -(-hy // Py)
+            nitem = nblk * nv * bxk
+            rsel, qsel = f"{res[-1][0]}", f"{res[-1][1]}"
+            for i in range(nv - 2, -1, -1):
+                rsel = f"(ri == {i} ? {res[i][0]} : {rsel})"
+                qsel = f"(ri == {i} ? {res[i][1]} : {qsel})"
+            w(f"        // class {k}: colour {c}, sublattice {a}, on the tile grown by {e} reaches")
+            w(f"        for (int t = threadIdx.x; t < {nitem}; t += {RT}) {{")
+            w(f"            const int u = t / {bxk}, j = t - u * {bxk};")
+            if nv == 1:
+                w(f"            const int ly = {y_lo + res[0][0]} + {Py} * u, q0 = {res[0][1]};")
+            else:
+                w(f"            const int m = u / {nv}, ri = u - m * {nv};")
+                w(f"            const int ly = {y_lo} + {Py} * m + {rsel}, q0 = {qsel};")
+            w(f"            const int lx = {x_lo} + q0 + {P} * j;")
+            w(f"            if (ly >= {y_lo + hy} || lx >= {x_lo + wx}) continue;")
+            w(f"            const int gxu = tx0 - {HX} + lx, gyu = ty0 - {HY} + ly;")
+            w(f"            {cx} {cy}")
+            conds = [v for v in (vx, vy) if v != "true"]
+            if conds:
+                w(f"            if (!({' && '.join(conds)})) continue;")
+            w(f"            const int own = {a * self.PL} + gy * {self.L1} + gx;")
+            if self.has_mask:
+                w("            if (!A.mask[own]) continue;")
+            w(f"            tupd_{a}(A, ld, {a * HW} + ly * {W} + lx, rep, own, key, beta_r, width_r);")
+            w("        }")
+            w("        __syncthreads();")
+        # write the tile back (its unwrapped cells inside the box: a partial tile's overhang is another tile's)
+        w(f"        float *D = (float *)A.dst + (int64_t)rep * {n * self.NS};")
+        w(f"        for (int i = threadIdx.x; i < {NST}; i += {RT}) {{")
+        w(f"            const int a = i / {TY * NCI}, r = i - a * {TY * NCI};")
+        w(f"            const int uy = r / {NCI}, c = r - uy * {NCI};")
+        w(f"            const int gx = tx0 + c * {V}, gy = ty0 + uy;")
+        w(f"            if (gx >= {self.L1} || gy >= {self.L2}) continue;")
+        w(f"            const float *sl = ld + {a_expr('a', HW)} + (uy + {HY}) * {W} + c * {V} + {HX};")
+        w(f"            const int64_t q = (int64_t)a * {self.PL} + (int64_t)gy * {self.L1} + gx;")
+        for k in range(n):
+            w(f"            *({VT} *)(D + {self.NS * k} + q) = *(const {VT} *)(sl + {k * NC});")
+        w("        }")
+        w("    }")
+        w("}")
+        w()
 
     # ---------------------------------------------------------------------------- resident kernel
     RESIDENT_THREADS = 256           # overridden per module (see __init__)
@@ -1716,10 +1983,16 @@ class _Gen:
                f"colouring {s.colouring.describe()}; mode {self.mode}; proposal {self.proposal}")
         self.w()
         rplan = self.resident_plan()
+        tplan = self.tiled_plan() if self.want_tiled else None
+        if self.want_tiled and tplan is None:
+            raise ValueError("the tiled sweep needs float32 vector spins on a 2-D lattice in the reference layout, "
+                             "larger than a tile and its halo")
         if self.mode in (MODE_INT8, MODE_F32):
             self.emit_plain()
             if rplan is not None:
                 self.emit_resident(rplan)
+            if tplan is not None:
+                self.emit_tiled(tplan)
         elif self.mode == MODE_PACKED:
             self.emit_packed()
         else:
@@ -1738,6 +2011,7 @@ class _Gen:
         kernels = [f"sck_class_{k}" for k in range(len(self.classes))] + ["sck_seq"] + \
                   [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"] + \
                   (["sck_resident"] if rplan is not None else []) + \
+                  (["sck_tiled"] if tplan is not None else []) + \
                   (["sck_sample"] if (self.n > 1 and self.mode == MODE_F32) else [])
         ki = {k: i for i, k in enumerate(kernels)}
         R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
@@ -1778,6 +2052,10 @@ class _Gen:
                      table_entries=total, ising_layout=layouts, rep_blocks=nblk)
         if rplan is not None:
             mod.resident = Launch(ki["sck_resident"], (self.B, 1, 1), (self.RESIDENT_THREADS, 1, 1))
+        if tplan is not None:
+            # persistent: grid.y = 0 asks the launcher for the workgroups resident per XCD (from the occupancy)
+            mod.tiled = Launch(ki["sck_tiled"], (XCDS, 0, 1), (self.TILE_THREADS, 1, 1))
+            mod.tile_dims = tplan[:2]
         mod.key = hashlib.sha256(src.encode()).hexdigest()[:16]
         return mod
 
@@ -2000,7 +2278,7 @@ def fits_resident(sys_: System, mode: int) -> bool:
 
 def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", learned_meta=None,
              replica_major: bool = False, rm_tile=None, rm_chunks=None, rm_sites=None,
-             resident_threads=None, rm_block=None) -> Module:
+             resident_threads=None, rm_block=None, tiled: bool = False, tile_dims=None) -> Module:
     """HIP source and launch plans of one system (the counterpart of SCK/generator.py:150 `source` and
     SCK/sampler.py:191-232 `_launch`)."""
     if proposal not in ("uniform", "cone", "learned"):
@@ -2008,4 +2286,9 @@ def generate(sys_: System, mode: int, batch: int, proposal: str = "uniform", lea
     if proposal != "uniform" and sys_.n == 1:
         raise ValueError("a cone proposal needs a vector model (n = 2 or 3)")
     return _Gen(as_system(sys_), mode, batch, proposal, learned_meta, replica_major, rm_tile, rm_chunks, rm_sites,
-                resident_threads, rm_block).generate()
+                resident_threads, rm_block, tiled, tile_dims).generate()
+
+
+def fits_tiled(sys_: System, mode: int, tile_dims=None) -> bool:
+    """Whether the tiled sweep kernel applies to the system (see _Gen.tiled_plan)."""
+    return _Gen(as_system(sys_), mode, 1, "uniform", tile_dims=tile_dims).tiled_plan() is not None

@@ -27,7 +27,7 @@ from . import _lib as L
 from . import hipgen as G
 from .system import System
 
-__all__ = ["Sampler", "COMPACT_MIN", "TILE", "MAX_BLOCK", "PTILE", "WORD", "BWG", "bitwise_check", "p_table",
+__all__ = ["Sampler", "choose_layout", "COMPACT_MIN", "TILE", "MAX_BLOCK", "PTILE", "WORD", "BWG", "bitwise_check", "p_table",
            "p_fixed"]
 
 COMPACT_MIN = 4
@@ -88,6 +88,30 @@ def p_fixed(sys_: System, beta: np.ndarray) -> np.ndarray:
     return np.clip(np.floor(_p64(sys_, beta) * 2.0 ** 32).astype(np.int64), 0, 2 ** 32)
 
 
+def choose_layout(system: System, mode: int, B: int, resident: bool = True, layout: str = "auto", tiled=None,
+                  tile_dims=None) -> tuple[str, bool]:
+    """(layout, tiled) a Sampler runs: "reference" or "replica" storage, and whether whole sweeps use the tiled
+    kernel. `resident` says whether the LDS-resident kernel may run (it takes precedence where it fits)."""
+    n, b = system.n, system.b
+    rm_ok = mode in (G.MODE_INT8, G.MODE_F32)
+    resident_ok = resident and G.fits_resident(system, mode)
+    tiled_ok = mode == G.MODE_F32 and not resident_ok and G.fits_tiled(system, mode, tile_dims)
+    if tiled and not tiled_ok:
+        raise ValueError("the tiled sweep needs float32 vector spins on a 2-D lattice larger than a tile and its "
+                         "halo (and no LDS-resident lattice)")
+    if tiled and layout == "replica":
+        raise ValueError("the tiled sweep runs the reference layout")
+    if tiled is None:
+        # whole sweeps in tiles where the replica-major lattice would exceed the cache, or where the batch is not
+        # whole waves of replicas: the lattice is then read about once per sweep instead of once per class
+        big = B * n * b * system.L3 * system.L2 * system.L1 * 4 > G.RM_BLOCK_BYTES
+        tiled = tiled_ok and layout != "replica" and (big or B % 64 != 0 or layout == "reference")
+    if layout == "auto":
+        # lanes over replicas need whole waves of replicas; a lattice that fits LDS runs resident instead
+        layout = "replica" if (rm_ok and B % 64 == 0 and not resident_ok and not tiled) else "reference"
+    return layout, bool(tiled)
+
+
 def _stream_ptr(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
@@ -98,7 +122,8 @@ class Sampler:
                  words_per_program: int | None = None, independent: bool = False, bitwise_tile: int | None = None,
                  bitwise_warps: int | None = None, max_block: int = MAX_BLOCK, proposal: str = "uniform",
                  width=None, learned=None, cache_dir=None, resident: bool = True, layout: str = "auto",
-                 tile=None, chunks=None, sites_per_block=None, resident_threads=None, replica_block=None):
+                 tile=None, chunks=None, sites_per_block=None, resident_threads=None, replica_block=None,
+                 tiled: bool | None = None, tile_dims=None):
         self.sys = system
         self.use_resident = bool(resident)
         self.bitwise = bool(bitwise)
@@ -154,11 +179,7 @@ class Sampler:
         rm_ok = self.mode in (G.MODE_INT8, G.MODE_F32)
         if layout == "replica" and not rm_ok:
             raise ValueError("the replica-major layout holds unpacked spins only")
-        if layout == "auto":
-            # lanes over replicas need whole waves of replicas; a lattice that fits LDS runs resident instead
-            layout = "replica" if (rm_ok and self.B % 64 == 0 and not (self.use_resident and
-                                                                       G.fits_resident(system, self.mode))) \
-                else "reference"
+        layout, self.use_tiled = choose_layout(system, self.mode, self.B, self.use_resident, layout, tiled, tile_dims)
         self.replica_major = layout == "replica"
         self.layout = layout
         # arguments are validated above, before the device (as the reference does); nothing runs on the host
@@ -170,7 +191,7 @@ class Sampler:
         self.module = G.generate(system, self.mode, self.B, proposal, self.learned_meta,
                                  replica_major=self.replica_major, rm_tile=tile, rm_chunks=chunks,
                                  rm_sites=sites_per_block, resident_threads=resident_threads,
-                                 rm_block=replica_block)
+                                 rm_block=replica_block, tiled=self.use_tiled, tile_dims=tile_dims)
         self.src = self.module.src
         dev = self.device
         m = system.site_mask()
@@ -238,6 +259,10 @@ class Sampler:
             rarr, _ = L.launches([mod.resident])
             self._keep.append(rarr)
             d.resident = C.cast(rarr, C.POINTER(L.LaunchC))
+        if mod.tiled is not None:
+            tarr, _ = L.launches([mod.tiled])
+            self._keep.append(tarr)
+            d.tiled = C.cast(tarr, C.POINTER(L.LaunchC))
         h = C.c_void_p()
         cache = str(cache_dir or L.CACHE).encode()
         with torch.cuda.device(self.device):
@@ -253,9 +278,20 @@ class Sampler:
                 pass
             self._handle = None
 
-    def _args(self, S: torch.Tensor, dst: torch.Tensor | None = None) -> L.Args:
+    def _workspace(self) -> torch.Tensor:
+        """The tiled sweep's second spin buffer (the sweeps alternate between it and the spins), allocated on
+        first use and kept with the sampler."""
+        if getattr(self, "_ws", None) is None:
+            s = self.sys
+            self._ws = torch.empty(self.B * s.n * s.nsite, dtype=torch.float32, device=self.device)
+        return self._ws
+
+    def _args(self, S: torch.Tensor, dst: torch.Tensor | None = None, sweeps: bool = False) -> L.Args:
+        """sck_args over S; `sweeps`: for whole coloured sweeps (the tiled sweep's workspace as dst)."""
         a = L.Args()
         a.spins = S.data_ptr()
+        if dst is None and sweeps and self.use_tiled:
+            dst = self._workspace()
         a.dst = dst.data_ptr() if dst is not None else None
         a.mask = self.mask.data_ptr()
         a.k1 = self.k1.data_ptr()
@@ -393,7 +429,7 @@ class Sampler:
         lo, hi = classes if classes is not None else (0, -1)
 
         def run(T):
-            a = self._args(T)
+            a = self._args(T, sweeps=True)
             with torch.cuda.device(self.device):
                 L.check(L.lib().sck_sweep(self._handle, C.byref(a), int(n), int(lo), int(hi),
                                           C.c_void_p(_stream_ptr(self.device))), "sweep")
@@ -407,7 +443,7 @@ class Sampler:
         self._check_spins(S)
         if not self._is_native(S):
             raise ValueError("timed sweeps need spins in the sampler's storage order (as init_spins returns them)")
-        a = self._args(S)
+        a = self._args(S, sweeps=True)
         ms, nl = C.c_float(), C.c_int32()
         with torch.cuda.device(self.device):
             L.check(L.lib().sck_sweep_timed(self._handle, C.byref(a), int(n), C.c_void_p(_stream_ptr(self.device)),
@@ -420,7 +456,7 @@ class Sampler:
         self._check_spins(S)
         if not self._is_native(S):
             raise ValueError("graph capture needs spins in the sampler's storage order (as init_spins returns them)")
-        a = self._args(S)
+        a = self._args(S, sweeps=True)
         g = C.c_void_p()
         with torch.cuda.device(self.device):
             torch.cuda.current_stream(self.device).synchronize()
