@@ -181,6 +181,17 @@ int sck_sync_sweep(sck_handle *h, const sck_args *args, int32_t n, int64_t spin_
 /* Float64 energy [B] and per-sublattice magnetization [B][b][n] into args->obs (SCK/sampler.py:326-345). */
 int sck_observe(sck_handle *h, const sck_args *args, void *stream);
 
+/* Bytes of the static device array behind sck_observe_noalloc (per GPU, held until the process ends). */
+#define SCK_OBS_SCRATCH_BYTES (32u << 20)
+
+/* sck_observe without a caller-provided workspace: the per-block partial sums go to a static device array of
+ * SCK_OBS_SCRATCH_BYTES in the library (args->obs_scratch is ignored), so the call allocates nothing (no hipMalloc,
+ * hipFree or allocating library routine). No counterpart in the reference; it is for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses that forbid allocation. Returns
+ * SCK_E_INVALID when the partial sums (batch x obs_chunks x (1 + b n) doubles) exceed that size. Every call on
+ * one GPU shares its array, which keeps the last call's contents; calls that overlap on one GPU race. */
+int sck_observe_noalloc(sck_handle *h, const sck_args *args, void *stream);
+
 /* Capture k sweeps (launches and counter bump; with observe != 0 an observation after them) in a HIP graph
  * (SCK/sampler.py:242-269). The graph reads the buffers of `args` by address. */
 int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t observe, sck_graph **out);

@@ -11,6 +11,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -260,6 +261,29 @@ int do_observe(sck_handle *h, const sck_args *args, hipStream_t s) {
     return SCK_OK;
 }
 
+// The partial sums of sck_observe_noalloc: a file-scope device array, one copy per GPU, which holds its
+// contents between calls (every call on a GPU shares it; calls that overlap on one GPU race). The host-side name is
+// only a handle: its device address is read once per device with hipGetSymbolAddress and used from then on.
+static __device__ __align__(256) double scratch[SCK_OBS_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int obs_scratch_ptr(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SCK_E_INVALID, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SCK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -417,6 +441,21 @@ int sck_sync_sweep(sck_handle *h, const sck_args *args, int32_t n, int64_t spin_
 int sck_observe(sck_handle *h, const sck_args *args, void *stream) {
     if (!h || !args || !args->obs || !args->obs_scratch) return fail(SCK_E_INVALID, "observe needs obs buffers");
     return do_observe(h, args, (hipStream_t)stream);
+}
+
+int sck_observe_noalloc(sck_handle *h, const sck_args *args, void *stream) {
+    if (!h || !args || !args->obs) return fail(SCK_E_INVALID, "observe needs the obs output buffer");
+    const size_t need = (size_t)h->batch * (size_t)h->obs_chunks * (size_t)(1 + h->b * h->n) * sizeof(double);
+    if (need > SCK_OBS_SCRATCH_BYTES)
+        return fail(SCK_E_INVALID, "observe_noalloc: the partial sums need " + std::to_string(need) +
+                                       " bytes, more than the library's static scratch of " +
+                                       std::to_string((size_t)SCK_OBS_SCRATCH_BYTES) + " bytes; use sck_observe");
+    double *scratch = nullptr;
+    int rc = obs_scratch_ptr(&scratch);
+    if (rc) return rc;
+    sck_args a = *args;
+    a.obs_scratch = scratch;
+    return do_observe(h, &a, (hipStream_t)stream);
 }
 
 int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t observe, sck_graph **out) {

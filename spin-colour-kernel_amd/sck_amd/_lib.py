@@ -23,6 +23,7 @@ SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
 SCK_ABI_VERSION = 4
+OBS_SCRATCH_BYTES = 32 << 20          # SCK_OBS_SCRATCH_BYTES of include/sck.h
 
 
 class Args(C.Structure):
@@ -54,9 +55,9 @@ class Desc(C.Structure):
 
 #: every symbol include/sck.h declares
 EXPORTS = ["sck_version", "sck_last_error", "sck_create", "sck_precompile", "sck_destroy", "sck_sweep",
-           "sck_sweep_timed", "sck_reference_sweep", "sck_sync_sweep", "sck_observe", "sck_graph_capture", "sck_graph_replay",
-           "sck_graph_destroy", "sck_time_replays", "sck_mlp_nparam", "sck_mlp_workspace_bytes",
-           "sck_mlp_train_step", "sck_mlp_table", "sck_sample_sites"]
+           "sck_sweep_timed", "sck_reference_sweep", "sck_sync_sweep", "sck_observe", "sck_observe_noalloc",
+           "sck_graph_capture", "sck_graph_replay", "sck_graph_destroy", "sck_time_replays", "sck_mlp_nparam",
+           "sck_mlp_workspace_bytes", "sck_mlp_train_step", "sck_mlp_table", "sck_sample_sites"]
 
 
 class MlpSample(C.Structure):
@@ -117,6 +118,7 @@ def lib():
         L.sck_reference_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_void_p]
         L.sck_sync_sweep.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int64, C.c_void_p]
         L.sck_observe.argtypes = [C.c_void_p, C.POINTER(Args), C.c_void_p]
+        L.sck_observe_noalloc.argtypes = [C.c_void_p, C.POINTER(Args), C.c_void_p]
         L.sck_graph_capture.argtypes = [C.c_void_p, C.POINTER(Args), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
         L.sck_graph_replay.argtypes = [C.c_void_p, C.c_void_p]
         L.sck_graph_destroy.argtypes = [C.c_void_p]
