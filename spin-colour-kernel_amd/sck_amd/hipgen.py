@@ -63,9 +63,9 @@ BW_WORDS = 2                  # independent bitwise mode: words per thread, shar
 LEVELS = 32
 RM_BLOCK_BYTES = 224 << 20    # replica-major: largest lattice share run through whole sweeps at a time (cache-resident)
 # tiled sweep: tile sizes (cells along x, y) tried in order, the first whose tile and halo fit the LDS budget.
-# XY triangular 2048^2 x 2048: 128 x 48 gives 1.58e11 site updates/s, 128 x 32 1.54e11, 64 x 64 1.52e11,
-# 96 x 64 1.50e11, 64 x 32 1.29e11 (one box, one run each)
-TILE_CANDIDATES = ((128, 48), (64, 64), (64, 48), (64, 32), (32, 32), (24, 24))
+# XY triangular 2048^2 x 2048: 128 x 32 gives 2.11e11 site updates/s, 64 x 64 2.04e11, 128 x 48 1.99e11,
+# 96 x 48 1.99e11 (one box, one run each)
+TILE_CANDIDATES = ((128, 32), (64, 64), (64, 48), (64, 32), (32, 32), (24, 24))
 
 
 @dataclass
@@ -245,6 +245,9 @@ typedef struct sck_args {
 } sck_args;
 
 #define DEV static __device__ __forceinline__
+
+// four floats as a native vector (registers, one 16-byte access)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Philox-4x32-10 (Salmon et al., SC'11): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
 // 0x9E3779B9 / 0xBB67AE85; one 32x32->64 multiply gives each product's high and low words.
@@ -1060,7 +1063,7 @@ class _Gen:
         ULD = -(-NLD // RT)                 # chunks per thread
         NCI = TX // V if V > 1 else TX
         NST = b * TY * NCI                  # chunks of the tile itself
-        VT = "float4" if V == 4 else "float"
+        VT = "f32x4" if V == 4 else "float"
         NC = b * HW                         # LDS cells per component: the tile is component-planar
         for a in range(b):
             offs = {}
@@ -1091,8 +1094,12 @@ class _Gen:
 
         cx, vx = ax("gxu", self.L1, self.per[0], "gx")
         cy, vy = ax("gyu", self.L2, self.per[1], "gy")
-        zero = "make_float4(0.0f, 0.0f, 0.0f, 0.0f)" if V == 4 else "0.0f"
-        w(f'extern "C" __global__ void __launch_bounds__({RT}) sck_tiled(sck_args A) {{')
+        zero = "(f32x4)(0.0f)" if V == 4 else "0.0f"
+        # the LDS of a tile bounds the workgroups per CU (one wave of each per SIMD): telling the compiler so lets
+        # it keep the prefetched tile in registers instead of trading them for an occupancy it cannot reach
+        wpe = max(1, min(8, (160 * 1024) // (b * HW * n * 4)))
+        w(f'extern "C" __global__ void __launch_bounds__({RT}) __attribute__((amdgpu_waves_per_eu(1, {wpe}))) '
+          f'sck_tiled(sck_args A) {{')
         w(f"    __shared__ __attribute__((aligned(16))) float ld[{b * HW * n}];")
         w(f"    const int64_t lo = (int64_t)blockIdx.x * {chunk};")
         w(f"    const int64_t hi = lo + {chunk} < {items}LL ? lo + {chunk} : {items}LL;")
@@ -1103,27 +1110,31 @@ class _Gen:
         # one named register (vector) per prefetched chunk and component, fully unrolled here: an indexed array
         # would be placed in scratch memory
         w(f"    {VT} {', '.join(f'p{u}_{k}' for u in range(ULD) for k in range(n))};")
-        # the loads of item gi (issued, not waited for)
-        w("    auto fetch = [&](int64_t gi) {")
-        w(f"        const int rep = (int)(gi / {NT}), tile = (int)(gi - (int64_t)rep * {NT});")
-        w(f"        const int tx0 = (tile % {NTX}) * {TX}, ty0 = (tile / {NTX}) * {TY};")
-        w(f"        const float *S = (const float *)A.spins + (int64_t)rep * {n * self.NS};")
-        for u in range(ULD):
-            w("        {")
-            w(f"            const int i = threadIdx.x + {u * RT};")
-            w(f"            const int a = i / {H * NCH}, r = i - a * {H * NCH};")
-            w(f"            const int ly = r / {NCH}, c = r - ly * {NCH};")
-            w(f"            const int gxu = tx0 - {HX} + c * {V}, gyu = ty0 - {HY} + ly;")
-            w(f"            {cx} {cy}")
-            w(f"            const bool ok = {'i < ' + str(NLD) + ' && ' if (u + 1) * RT > NLD else ''}{vx} && {vy};")
-            # an always-valid address, so the loads issue without branches; cells outside the box (and the
-            # padding chunks) are zeroed or skipped when written to LDS, so nothing waits for these loads before then
-            w(f"            const int64_t q = ok ? (int64_t)a * {self.PL} + (int64_t)gy * {self.L1} + gx : 0;")
-            for k in range(n):
-                w(f"            p{u}_{k} = *(const {VT} *)(S + {self.NS * k} + q);")
-            w("        }")
-        w("    };")
-        w("    fetch(g);")
+        def fetch(item: str, ind: str) -> None:
+            """The loads of item `item` into p<u>_<k> (issued, not waited for). Emitted in place: a lambda called
+            twice is compiled by hipRTC as a call whose closure lives in scratch memory."""
+            w(f"{ind}{{")
+            w(f"{ind}    const int64_t gi = {item};")
+            w(f"{ind}    const int rep = (int)(gi / {NT}), tile = (int)(gi - (int64_t)rep * {NT});")
+            w(f"{ind}    const int tx0 = (tile % {NTX}) * {TX}, ty0 = (tile / {NTX}) * {TY};")
+            w(f"{ind}    const float *S = (const float *)A.spins + (int64_t)rep * {n * self.NS};")
+            for u in range(ULD):
+                w(f"{ind}    {{")
+                w(f"{ind}        const int i = threadIdx.x + {u * RT};")
+                w(f"{ind}        const int a = i / {H * NCH}, r = i - a * {H * NCH};")
+                w(f"{ind}        const int ly = r / {NCH}, c = r - ly * {NCH};")
+                w(f"{ind}        const int gxu = tx0 - {HX} + c * {V}, gyu = ty0 - {HY} + ly;")
+                w(f"{ind}        {cx} {cy}")
+                w(f"{ind}        const bool ok = {'i < ' + str(NLD) + ' && ' if (u + 1) * RT > NLD else ''}{vx} && {vy};")
+                # an always-valid address, so the loads issue without branches; cells outside the box (and the
+                # padding chunks) are zeroed or skipped when written to LDS, so nothing waits for these loads here
+                w(f"{ind}        const int64_t q = ok ? (int64_t)a * {self.PL} + (int64_t)gy * {self.L1} + gx : 0;")
+                for k in range(n):
+                    w(f"{ind}        p{u}_{k} = *(const {VT} *)(S + {self.NS * k} + q);")
+                w(f"{ind}    }}")
+            w(f"{ind}}}")
+
+        fetch("g", "    ")
         w("    for (; g < hi; g += step) {")
         w(f"        const int rep = (int)(g / {NT}), tile = (int)(g - (int64_t)rep * {NT});")
         w(f"        const int tx0 = (tile % {NTX}) * {TX}, ty0 = (tile / {NTX}) * {TY};")
@@ -1158,7 +1169,8 @@ class _Gen:
                 w("            }")
             w("        }")
         w("        __syncthreads();")
-        w("        if (g + step < hi) fetch(g + step);          // in flight during the class passes")
+        w("        if (g + step < hi)                         // in flight during the class passes")
+        fetch("g + step", "        ")
         P, Py = self.P, y_period(col)
         self.tiled_tables = []
         for k, (c, a) in enumerate(self.classes):
@@ -1215,8 +1227,10 @@ class _Gen:
         w(f"            if (gx >= {self.L1} || gy >= {self.L2}) continue;")
         w(f"            const float *sl = ld + {a_expr('a', HW)} + (uy + {HY}) * {W} + c * {V} + {HX};")
         w(f"            const int64_t q = (int64_t)a * {self.PL} + (int64_t)gy * {self.L1} + gx;")
+        # non-temporal: the written tile is not read again in this sweep, so its lines do not displace the halo
+        # rows that the next tiles read from L2 (+2 % at XY 2048^2 x 2048, one box)
         for k in range(n):
-            w(f"            *({VT} *)(D + {self.NS * k} + q) = *(const {VT} *)(sl + {k * NC});")
+            w(f"            __builtin_nontemporal_store(*(const {VT} *)(sl + {k * NC}), ({VT} *)(D + {self.NS * k} + q));")
         w("        }")
         w("    }")
         w("}")

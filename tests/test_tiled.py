@@ -56,10 +56,10 @@ def test_tiled_plan_geometry():
     s = sy.xy("triangular", 2048)
     g = G._Gen(s, G.MODE_F32, 2048, "uniform")
     assert len(g.classes) == 4                     # 2048 mod 3 != 0: the seam condition forces four colours
-    assert g.tiled_plan() == (128, 48, 4, 4, 1, 1)
+    assert g.tiled_plan() == (128, 32, 4, 4, 1, 1)
     assert G._Gen(s, G.MODE_F32, 2048, "uniform", tile_dims=(64, 64)).tiled_plan() == (64, 64, 4, 4, 1, 1)
-    # Heisenberg: 128 x 48 would not fit the LDS budget, so the next candidate is taken
-    assert G._Gen(sy.heisenberg("triangular", 300), G.MODE_F32, 8, "uniform").tiled_plan()[:2] == (60, 63)
+    # Heisenberg triangular, three colours: the tile is rounded to the colouring period and to float4 columns
+    assert G._Gen(sy.heisenberg("triangular", 300), G.MODE_F32, 8, "uniform").tiled_plan()[:2] == (120, 30)
     assert not G.fits_tiled(sy.xy("triangular", 12), G.MODE_F32)          # smaller than a tile and its halo
     assert not G.fits_tiled(sy.heisenberg("bcc", 32), G.MODE_F32)         # 3-D
     assert not G.fits_tiled(sy.ising("square", 256), G.MODE_INT8)         # Ising
