@@ -265,7 +265,7 @@ class BatchSampler:
         if proposal == "cone":
             w = np.full(self.R, np.pi, np.float32) if width is None else np.broadcast_to(
                 np.asarray(width, np.float32).reshape(-1), (self.R,))
-            self.width = torch.as_tensor(np.ascontiguousarray(w), device=dev)
+            self.width = torch.as_tensor(np.array(w, np.float32), device=dev)
         elif proposal == "learned":
             if learned is None:
                 raise ValueError("proposal='learned' needs a trained LearnedWidth")
@@ -439,7 +439,7 @@ class BatchSampler:
         w = np.broadcast_to(np.asarray(width, np.float32).reshape(-1), (self.R,))
         if np.any(w <= 0) or np.any(w > np.pi + 1e-6):
             raise ValueError("cone half-angles must lie in (0, pi]")
-        self.width.copy_(torch.as_tensor(np.ascontiguousarray(w), device=self.device))
+        self.width.copy_(torch.as_tensor(np.array(w, np.float32), device=self.device))
 
     # ----------------------------------------------------------------------------------- sweeps
     def sweep(self, S: torch.Tensor, n: int = 1, classes: tuple[int, int] | None = None) -> torch.Tensor:

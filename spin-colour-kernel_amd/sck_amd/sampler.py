@@ -213,7 +213,7 @@ class Sampler:
         if proposal == "cone":
             w = np.full(self.Bp, np.pi, np.float32) if width is None else np.broadcast_to(
                 np.asarray(width, np.float32).reshape(-1), (self.Bp,))
-            self.width = torch.as_tensor(np.ascontiguousarray(w), device=dev)
+            self.width = torch.as_tensor(np.array(w, np.float32), device=dev)
         elif proposal == "learned":
             if learned is None:
                 raise ValueError("proposal='learned' needs a trained LearnedWidth (sck_amd.proposal.train_width)")
@@ -377,7 +377,7 @@ class Sampler:
             w = np.concatenate([w, np.full(self.Bp - self.B, w[-1], np.float32)])
         if np.any(w <= 0) or np.any(w > np.pi + 1e-6):
             raise ValueError("cone half-angles must lie in (0, pi]")
-        self.width.copy_(torch.as_tensor(np.ascontiguousarray(w), device=self.device))
+        self.width.copy_(torch.as_tensor(np.array(w, np.float32), device=self.device))
 
     def set_seed(self, seed: int, sweep: int = 0):
         self.state[0] = int(seed)

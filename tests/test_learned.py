@@ -243,3 +243,21 @@ def test_single_xy_spin_distribution_is_exact(kind):
     est = float(np.mean(acc))
     err = float(np.std(acc)) / math.sqrt(len(acc)) * 3 + 2e-3
     assert abs(est - exact) < err, (kind, est, exact)
+
+
+@pytest.mark.gpu
+def test_learned_width_gains_at_the_cold_end():
+    """Heisenberg bcc J1-J2 + Du + h on 8^3 cells, B = 256 at T = 0.5 and 0.6: the trained width decorrelates the
+    energy faster than the uniform redraw (the reference's proposal) at the coldest temperature, where small moves
+    are accepted and large ones are not: tau_int(E) of the learned width at most 0.75 of the uniform redraw's
+    (measured 0.4 at the Heisenberg bcc 32^3 configuration, where the ESS/s gain at T = 0.5 is 2.5x). The
+    observables of both agree within the jackknife errors."""
+    from sck_amd.ess import compare_proposals
+    s = sy.System("Heis bcc J1J2 D h", 3, cr.make("bcc", (1, 2)), (8, 8, 8), (True,) * 3, {1: 1.0, 2: -0.2},
+                  h=(0.0, 0.0, 0.05), uniaxial=(0.1, (0.0, 0.0, 1.0)))
+    r = compare_proposals(s, 256, [0.5, 0.6], warmup=100, equil=300, therm=20, n_meas=400, train_steps=200,
+                          widths=(0.5, 1.0), log=lambda m: print(m, flush=True))
+    t_u, t_l = r["uniform"]["tau_E"][0], r["learned"]["tau_E"][0]
+    print(f"tau_int(E) at T = 0.5: uniform {t_u:.2f}, learned {t_l:.2f}", flush=True)
+    assert t_l < 0.75 * t_u
+    assert r["max_z"] < 4.0
