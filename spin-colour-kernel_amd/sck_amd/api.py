@@ -1,24 +1,25 @@
-"""The short interface: describe, check and run a simulation, then read its results (SCK/api.py:1-226).
+"""The short interface of a simulation: describe, check and run it, then read its results (the reference's
+`sck.describe/check/run/load/Results`, SCK/api.py:36-226).
 
     from sck_amd import api as sck
-    sck.describe()                                # models, crystals, parameters, defaults
-    sck.check(crystal="sc", box=[16, 16, 16])     # errors and warnings, without running
+    sck.describe()                                # models, crystals, state parameters, defaults
+    sck.check(crystal="sc", box=[16, 16, 16])     # errors and warnings, nothing runs
     res = sck.run(crystal="sc", box=[16, 16, 16], states=[dict(name="ferromagnet", J={1: 1.0})],
                   T_max=2.0, T_min=0.5, T_step=0.1)
     res.table()                                   # E, |M|, Cv and chi per state and temperature
 
-`run` uses every visible MI355X: one GPU runs in this process; several run one process each
-(`python -m sck_amd.simulate`), each with an equal block of the replicas of every state. There is no CPU path.
-It writes the reference's output folder and returns its results.
+A configuration is a dict and/or keyword settings (keywords win); settings left out take the defaults of
+`describe()["defaults"]`. `run` uses the visible MI355X GPUs: one runs in this process, several run one process
+each (`python -m sck_amd.simulate`) on an equal block of every state's replicas. There is no CPU path. The output
+folder is the reference's format (sck_amd.simulate).
 """
 
 from __future__ import annotations
 
-import json
 import os
+import selectors
 import subprocess
 import sys
-import threading
 import time
 from pathlib import Path
 
@@ -30,193 +31,220 @@ from .hamiltonian import MODELS, build_system
 
 __all__ = ["describe", "check", "run", "load", "Results"]
 
-_PACKAGE_ROOT = str(Path(__file__).resolve().parents[1])
+_PKG_PARENT = str(Path(__file__).resolve().parents[1])
 _STATE_KEYS = ("name", "J", "D", "K1", "Ks", "K2", "H", "H_theta", "H_phi")
+_RUN_KEYS = tuple(SIM.default_config())            # the settings a run folder records
+_EXTRA_KEYS = ("output_dir", "gpus")               # settings of `run` itself
 
 
 def describe() -> dict:
     """The models, crystals, coordinate systems, state parameters, units and every default setting."""
-    return dict(models=list(MODELS), crystals=sorted(cr.KNOWN), coordinates={k: list(v) for k, v in SIM.COORDS.items()},
-                state_parameters=list(_STATE_KEYS), units=["reduced", "meV"], defaults=SIM.default_config())
+    return {"models": list(MODELS), "crystals": sorted(cr.KNOWN),
+            "coordinates": {k: list(v) for k, v in SIM.COORDS.items()}, "state_parameters": list(_STATE_KEYS),
+            "units": ["reduced", "meV"], "defaults": SIM.default_config()}
 
 
-def _merge(config: dict | None, settings: dict) -> dict:
-    cfg = dict(config or {})
-    cfg.update(settings)
-    return cfg
+def _settings(config, extra) -> dict:
+    """Defaults, then the configuration dict, then keyword settings."""
+    return {**SIM.default_config(), **dict(config or {}), **extra}
 
 
-def _simulation_config(full: dict) -> dict:
-    keys = ("model", "crystal", "box", "periodic", "coords", "shape", "states", "units", "T_max", "T_min", "T_step",
-            "thermalization", "measurement", "measure_every", "replicas", "config_every", "seed")
-    cfg = {k: full[k] for k in keys}
+def _normalised(cfg: dict) -> dict:
+    """The run's own settings, with every state named ("state-<k>" by default) and J keyed by shell strings."""
+    sim = {k: cfg[k] for k in _RUN_KEYS}
     states = []
-    for i, s in enumerate(cfg["states"]):
-        s = dict(s)
-        s.setdefault("name", f"state-{i + 1}")
-        s["J"] = {str(k): float(v) for k, v in (s.get("J") or {1: 1.0}).items()}
-        states.append(s)
-    cfg["states"] = states
-    return cfg
+    for k, st in enumerate(sim["states"], start=1):
+        st = dict(st)
+        st.setdefault("name", f"state-{k}")
+        st["J"] = {str(shell): float(v) for shell, v in (st.get("J") or {1: 1.0}).items()}
+        states.append(st)
+    sim["states"] = states
+    return sim
+
+
+class _Issues:
+    def __init__(self):
+        self.errors: list[str] = []
+        self.warnings: list[str] = []
+
+    def attempt(self, fn, prefix: str = ""):
+        """fn() with its ValueError/KeyError recorded as an error; None then."""
+        try:
+            return fn()
+        except (ValueError, KeyError, TypeError) as e:
+            self.errors.append(f"{prefix}{e}")
+            return None
+
+
+def _rule_ladder(sim, issues):
+    temps = issues.attempt(lambda: SIM.temperature_ladder(sim["T_max"], sim["T_min"], sim["T_step"]))
+    if temps is not None and len(temps) > 1000:
+        issues.warnings.append(f"{len(temps)} temperatures")
+    issues.attempt(lambda: SIM._kB(sim["units"]))
+
+
+def _rule_counts(sim, issues):
+    for key, least in (("thermalization", 0), ("measurement", 1), ("measure_every", 1), ("replicas", 1),
+                       ("config_every", 1)):
+        ok = issues.attempt(lambda: int(sim[key]) >= least)
+        if ok is False:
+            issues.errors.append(f"{key} must be a positive integer")
+
+
+def _rule_names(sim, issues):
+    names = [st["name"] for st in sim["states"]]
+    if not names:
+        issues.errors.append("define at least one state")
+    elif len(set(names)) < len(names):
+        issues.errors.append("state names must be distinct")
+
+
+def _rule_states(sim, issues, report):
+    found = issues.attempt(lambda: SIM.active_sites(sim["crystal"], sim["box"], sim["coords"], sim["shape"]))
+    if found is None:
+        return
+    mask = found[1]
+    for k, theta in enumerate(sim["states"]):
+        s = issues.attempt(lambda: build_system(sim["model"], sim["crystal"], sim["box"], sim["periodic"], theta,
+                                                mask, log=lambda m: issues.warnings.append(m.strip())),
+                           prefix=f"state {k}: ")
+        if s is not None and k == 0:
+            report.update(n_sites=int(mask.sum()), colouring=s.colouring.describe())
 
 
 def check(config: dict | None = None, **settings) -> dict:
-    """Validate a configuration without running it: {"ok", "errors", "warnings", "config"}; never raises for a
-    bad configuration (the errors come back)."""
-    cfg = {**SIM.default_config(), **_merge(config, settings)}
-    errors, warnings = [], []
-    unknown = set(cfg) - set(SIM.default_config()) - {"output_dir", "gpus"}
+    """Validate a configuration without running it: {"ok", "errors", "warnings", "config"}. A bad configuration
+    comes back as errors; it never raises."""
+    cfg = _settings(config, settings)
+    issues = _Issues()
+    unknown = sorted(set(cfg) - set(_RUN_KEYS) - set(_EXTRA_KEYS))
     if unknown:
-        errors.append(f"unknown settings {sorted(unknown)}")
-    try:
-        full = _simulation_config(cfg)
-    except Exception as e:          # noqa: BLE001 - every failure becomes an error message
-        return dict(ok=False, errors=errors + [str(e)], warnings=warnings, config=cfg)
-    try:
-        temps = SIM.temperature_ladder(full["T_max"], full["T_min"], full["T_step"])
-        if len(temps) > 1000:
-            warnings.append(f"{len(temps)} temperatures")
-    except ValueError as e:
-        errors.append(str(e))
-    try:
-        SIM._kB(full["units"])
-    except ValueError as e:
-        errors.append(str(e))
-    for k in ("thermalization", "measurement", "measure_every", "replicas", "config_every"):
-        if int(full[k]) < (0 if k == "thermalization" else 1):
-            errors.append(f"{k} must be a positive integer")
-    names = [s["name"] for s in full["states"]]
-    if not names:
-        errors.append("define at least one state")
-    if len(set(names)) != len(names):
-        errors.append("state names must be distinct")
-    mask = None
-    try:
-        _, mask = SIM.active_sites(full["crystal"], full["box"], full["coords"], full["shape"])
-    except (ValueError, KeyError) as e:
-        errors.append(str(e))
-    if mask is not None:
-        for i, th in enumerate(full["states"]):
-            try:
-                s = build_system(full["model"], full["crystal"], full["box"], full["periodic"], th, mask,
-                                 log=lambda m: warnings.append(m.strip()))
-                if i == 0:
-                    cfg["n_sites"], cfg["colouring"] = int(mask.sum()), s.colouring.describe()
-            except (ValueError, KeyError) as e:
-                errors.append(f"state {i}: {e}")
-    return dict(ok=not errors, errors=errors, warnings=warnings, config={**cfg, **full})
+        issues.errors.append(f"unknown settings {unknown}")
+    sim = issues.attempt(lambda: _normalised(cfg))
+    if sim is None:
+        return dict(ok=False, errors=issues.errors, warnings=issues.warnings, config=cfg)
+    report = {}
+    for rule in (_rule_ladder, _rule_counts, _rule_names):
+        rule(sim, issues)
+    _rule_states(sim, issues, report)
+    return dict(ok=not issues.errors, errors=issues.errors, warnings=issues.warnings,
+                config={**cfg, **report, **sim})
 
 
-def _devices(gpus) -> list[int]:
+def _visible_gpus(wanted) -> list[int]:
     import torch
     if not torch.cuda.is_available():
         return []
-    usable = list(range(torch.cuda.device_count()))
-    if gpus not in (None, "all"):
-        wanted = [gpus] if isinstance(gpus, int) else list(gpus)
-        usable = [g for g in usable if g in wanted]
-    return usable
+    ids = range(torch.cuda.device_count())
+    if wanted in (None, "all"):
+        return list(ids)
+    keep = {wanted} if isinstance(wanted, int) else set(wanted)
+    return [g for g in ids if g in keep]
 
 
 def run(config: dict | None = None, *, output_dir=None, gpus=None, verbose: bool = True, **settings) -> "Results":
-    """Run a simulation on every usable GPU and return its results; refused before anything starts when the
-    configuration has errors."""
-    cfg = _merge(config, settings)
+    """Check a configuration, run it on the usable GPUs and return its results. Nothing starts when the check
+    reports errors (ValueError); without a HIP device it raises RuntimeError."""
+    extra = dict(settings)
     if output_dir is not None:
-        cfg["output_dir"] = str(output_dir)
+        extra["output_dir"] = str(output_dir)
     if gpus is not None:
-        cfg["gpus"] = gpus
-    r = check(cfg)
-    if not r["ok"]:
-        raise ValueError("the configuration has errors: " + "; ".join(r["errors"]))
-    full = r["config"]
+        extra["gpus"] = gpus
+    report = check(config, **extra)
+    if not report["ok"]:
+        raise ValueError("the configuration has errors: " + "; ".join(report["errors"]))
+    cfg = report["config"]
     say = print if verbose else (lambda *a, **k: None)
-    for w in r["warnings"]:
+    for w in report["warnings"]:
         say("warning:", w)
-    usable = _devices(full.get("gpus", "all"))
-    if not usable:
+    devices = _visible_gpus(cfg.get("gpus", "all"))
+    if not devices:
         raise RuntimeError("no HIP device is visible: the MI355X backend has no CPU path")
-    out = Path(full.get("output_dir") or "simulation_output")
-    sim_cfg = _simulation_config(full)
-    prepared = SIM.prepare(sim_cfg, out, log=say)
-    say(f"{len(prepared['temperatures'])} temperatures from {prepared['temperatures'][0]} to "
-        f"{prepared['temperatures'][-1]}; {len(usable)} GPU(s): {usable}")
-    t0 = time.perf_counter()
-    n = len(sim_cfg["states"])
-    if len(usable) == 1:
-        SIM.run(prepared, list(range(n)), out, device=f"cuda:{usable[0]}", log=say)
+    out = Path(cfg.get("output_dir") or "simulation_output")
+    prepared = SIM.prepare(_normalised(cfg), out, log=say)
+    T = prepared["temperatures"]
+    say(f"{len(T)} temperatures from {T[0]} to {T[-1]}; {len(devices)} GPU(s): {devices}")
+    start = time.perf_counter()
+    states = list(range(len(prepared["states"])))
+    if len(devices) == 1:
+        SIM.run(prepared, states, out, device=f"cuda:{devices[0]}", log=say)
     else:
-        _run_on_gpus(out, n, int(full["replicas"]), usable, say)
-    say(f"total: {(time.perf_counter() - t0) / 60:.2f} min")
+        _one_process_per_gpu(out, states, int(prepared["replicas"]), devices, say)
+    say(f"total: {(time.perf_counter() - start) / 60:.2f} min")
     return load(out)
 
 
-def _run_on_gpus(out: Path, n_states: int, replicas: int, usable: list[int], say) -> None:
-    """One process per GPU, each with an equal block of every state's replicas (the reference's layout)."""
-    blocks = SIM.replica_blocks(replicas, len(usable))
-    every = ",".join(map(str, range(n_states)))
-    procs = []
-    for part, (g, (r0, r1)) in enumerate(zip(usable, blocks)):
-        env = dict(os.environ, HIP_VISIBLE_DEVICES=str(g),
-                   PYTHONPATH=os.pathsep.join(p for p in (_PACKAGE_ROOT, os.environ.get("PYTHONPATH", "")) if p))
-        cmd = [sys.executable, "-u", "-m", "sck_amd.simulate", "--run", str(out), "--states", every,
-               "--replicas", f"{r0}:{r1}", "--part", str(part), "--device", "cuda:0"]
-        procs.append((g, subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+def _one_process_per_gpu(out: Path, states, replicas: int, devices: list[int], say) -> None:
+    """One `python -m sck_amd.simulate` process per GPU, each on its replica block of every state, started with
+    only that GPU visible; their output lines are relayed as they come, prefixed by the GPU."""
+    path = os.pathsep.join(p for p in (_PKG_PARENT, os.environ.get("PYTHONPATH", "")) if p)
+    sel = selectors.DefaultSelector()
+    procs = {}
+    for part, (g, (r0, r1)) in enumerate(zip(devices, SIM.replica_blocks(replicas, len(devices)))):
+        cmd = [sys.executable, "-u", "-m", "sck_amd.simulate", "--run", str(out),
+               "--states", ",".join(map(str, states)), "--replicas", f"{r0}:{r1}", "--part", str(part),
+               "--device", "cuda:0"]
+        p = subprocess.Popen(cmd, env={**os.environ, "HIP_VISIBLE_DEVICES": str(g), "PYTHONPATH": path},
+                             stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        procs[g] = p
+        sel.register(p.stdout, selectors.EVENT_READ, g)
         say(f"GPU {g}: replicas {r0}-{r1 - 1} of every state")
-
-    def pump(g, p):
-        for line in p.stdout:
-            say(f"GPU {g} {line}", end="", flush=True)
-    threads = [threading.Thread(target=pump, args=gp) for gp in procs]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    bad = [g for g, p in procs if p.wait() != 0]
-    if bad:
-        raise RuntimeError(f"the run failed on GPU(s) {bad}: read the lines printed above")
+    open_streams = len(procs)
+    while open_streams:
+        for key, _ in sel.select():
+            line = key.fileobj.readline()
+            if line:
+                say(f"GPU {key.data} {line}", end="", flush=True)
+            else:
+                sel.unregister(key.fileobj)
+                open_streams -= 1
+    failed = [g for g, p in procs.items() if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"the run failed on GPU(s) {failed}: read the lines printed above")
 
 
 def load(output_dir="simulation_output") -> "Results":
-    """The results of a finished run, read from its output folder (the per-GPU files are combined once)."""
-    out = Path(output_dir)
-    data = SIM.collect(out)
-    return Results(out, json.loads((out / "run.json").read_text()), data)
+    """The results of a finished run, read from its folder (the per-GPU files are combined on the way)."""
+    folder = SIM.RunFolder(output_dir)
+    return Results(folder.path, folder.config(), SIM.collect(folder.path))
 
 
 class Results:
-    """A finished run: per state and temperature, E and |M| per site, Cv and chi, and the output folder."""
+    """A finished run. Per state (rows) and temperature (columns): energy_per_site, magnetization_per_site,
+    specific_heat, susceptibility. Also names, temperatures, config (run.json), arrays (results.npz) and
+    output_dir; configurations() maps the saved spins without reading them."""
 
     def __init__(self, output_dir: Path, config: dict, data: dict):
-        self.output_dir, self.config, self.arrays = Path(output_dir), config, data
-        self.names = [str(n) for n in data["state_names"]]
+        self.output_dir = Path(output_dir)
+        self.config = config
+        self.arrays = data
+        self.names = [str(v) for v in data["state_names"]]
         self.temperatures = data["temperatures"]
-        self.energy_per_site = data["E_site"]
-        self.magnetization_per_site = data["M_site"]
-        self.specific_heat = data["Cv"]
-        self.susceptibility = data["chi"]
+
+    energy_per_site = property(lambda self: self.arrays["E_site"])
+    magnetization_per_site = property(lambda self: self.arrays["M_site"])
+    specific_heat = property(lambda self: self.arrays["Cv"])
+    susceptibility = property(lambda self: self.arrays["chi"])
 
     @property
     def temperature_unit(self) -> str:
-        return "K" if str(self.config.get("units", "")).lower() == "mev" else "J/k_B"
+        return "K" if str(self.config.get("units", "")).strip().lower() == "mev" else "J/k_B"
 
     def configurations(self):
-        """The saved spins, memory-mapped: (states, replicas, saved T, sites, components)."""
-        return np.load(self.output_dir / "configurations.npy", mmap_mode="r")
+        """The saved spins, memory-mapped: (states, replicas, saved temperatures, sites, 3)."""
+        return SIM.RunFolder(self.output_dir).configurations()
 
     def table(self) -> list[dict]:
-        """One row per state and temperature."""
-        return [dict(state=n, T=float(t), E_site=float(self.energy_per_site[i, k]),
-                     M_site=float(self.magnetization_per_site[i, k]), Cv=float(self.specific_heat[i, k]),
-                     chi=float(self.susceptibility[i, k]))
-                for i, n in enumerate(self.names) for k, t in enumerate(self.temperatures)]
+        """One row per (state, temperature)."""
+        cols = (("E_site", self.energy_per_site), ("M_site", self.magnetization_per_site),
+                ("Cv", self.specific_heat), ("chi", self.susceptibility))
+        return [{"state": name, "T": float(T), **{k: float(v[i, j]) for k, v in cols}}
+                for i, name in enumerate(self.names) for j, T in enumerate(self.temperatures)]
 
     def __repr__(self) -> str:
-        T = self.temperatures
-        rows = [f"Results({self.output_dir}): {len(self.names)} state(s), {len(T)} temperatures "
-                f"{T[0]:g} to {T[-1]:g} {self.temperature_unit}"]
-        for i, n in enumerate(self.names):
-            rows.append(f"  {n}: at T = {T[-1]:g}  E/site {self.energy_per_site[i, -1]:.5f}  "
-                        f"|M| {self.magnetization_per_site[i, -1]:.4f}  Cv {self.specific_heat[i, -1]:.4f}  "
-                        f"chi {self.susceptibility[i, -1]:.4f}")
-        return "\n".join(rows)
+        T, u = self.temperatures, self.temperature_unit
+        head = f"Results({self.output_dir}): {len(self.names)} state(s), {len(T)} temperatures {T[0]:g} to {T[-1]:g} {u}"
+        last = [f"  {name}: T = {T[-1]:g}  E/site {self.energy_per_site[i, -1]:.5f}  "
+                f"|M| {self.magnetization_per_site[i, -1]:.4f}  Cv {self.specific_heat[i, -1]:.4f}  "
+                f"chi {self.susceptibility[i, -1]:.4f}" for i, name in enumerate(self.names)]
+        return "\n".join([head] + last)

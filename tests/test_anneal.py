@@ -123,3 +123,24 @@ def test_main_writes_one_file_per_state(tmp_path):
     assert z["obs"].shape == (3, 8, 8) and z["M"].shape == (8, 3, 3) and z["snaps"].shape[0] == 3
     assert np.allclose(z["temperatures"], [60.0, 32.5, 5.0]) and int(z["n_active"]) > 0
     assert np.array_equal(z["params"], np.asarray([0.0] * 3 + ROW[3:], np.float32))
+
+
+@pytest.mark.gpu
+def test_anneal_ladder_matches_reference():
+    """A nanodot cooling ladder (Rd = 2, two layers, 30 degree field, two temperatures) against the accumulators,
+    magnetization vectors, middle-layer snapshots and reduced observables the reference's anneal_state and
+    reduce_obs produced on its CPU path (SCK/anneal.py:143-234), from the same seed: fp32 decision tolerance."""
+    from pathlib import Path
+
+    from make_golden import ANNEAL_CASE as c
+    from sck_amd import anneal as AN
+    z = np.load(Path(__file__).resolve().parent / "golden" / "anneal_nanodot.npz")
+    s = AN.build_state(c["params"], c["Rd"], c["L"], c["angle"], (1,))
+    assert s.n_active == int(z["n_active"])
+    r = AN.anneal_state(s, c["replicas"], c["temps_K"], c["n_therm"], c["n_meas"], c["every"], c["seed"],
+                        device="cuda:0", log=lambda m: None)
+    assert np.allclose(r["obs"], z["obs"], rtol=1e-4, atol=1e-4)
+    assert np.allclose(r["mvec"], z["mvec"], atol=1e-5) and np.allclose(r["snaps"], z["snaps"], atol=1e-5)
+    red = AN.reduce_obs(r["obs"], c["temps_K"], s.n_active)
+    for k in ("E", "M", "Mz", "Cv", "chi", "Q"):
+        assert np.allclose(red[k], z[f"reduced/{k}"], rtol=1e-4, atol=1e-4), k
