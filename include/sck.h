@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SCK_ABI_VERSION 4
+#define SCK_ABI_VERSION 5
 
 enum sck_status {
     SCK_OK = 0,
@@ -62,6 +62,10 @@ typedef struct sck_args {
     const int32_t *tiles;     /* batched systems: the launch's work list (set from sck_launch)     */
     int32_t ntile;            /* its length                                                        */
     int32_t rep_base;         /* first replica of the launch's replica block (set from sck_launch) */
+    double *track;            /* NULL, or running float64 totals E [B] then M [B][b][n], kept current by whole
+                                 sweeps of the tiled kernel (which add the energy and magnetization changes of
+                                 their accepted moves); the caller anchors them with sck_observe            */
+    double *track_parts;      /* with track: [tiles][B][1 + b n] per-(replica, tile) partial sums of one sweep */
 } sck_args;
 
 /* One kernel launch of a plan. */
@@ -101,6 +105,8 @@ typedef struct sck_desc {
                                                     args.spins and writing args.dst. Whole sweeps use it when
                                                     args.dst is a workspace of the spins' size (unless
                                                     `resident` is set); otherwise the class launches run */
+    const sck_launch *track_fold;                /* with `tiled`: adds one sweep's partial sums (args.track_parts)
+                                                    to the running totals args.track                            */
 } sck_desc;
 
 /* ---------------------------------------------------------------- learned proposal width (no reference
@@ -159,7 +165,10 @@ int sck_destroy(sck_handle *h);
 /* n coloured sweeps in place, then state[1] += n (SCK/sampler.py:234-240). stream: a hipStream_t or NULL.
  * class_lo/class_hi select launches [lo, hi) of one sweep (0, -1: all); used by per-class checks.
  * A module with a tiled launch runs whole sweeps through it when args->dst points to a workspace of the spins'
- * size: the sweeps alternate between the two buffers and the result ends in args->spins. */
+ * size: the sweeps alternate between the two buffers and the result ends in args->spins.
+ * With args->track set, the sweeps must run tiled (whole sweeps, args->dst given), and each adds the energy and
+ * magnetization changes of its accepted moves to args->track: float64 totals that stay equal to sck_observe's
+ * up to the fp32 rounding of each move's dE (SCK_E_INVALID for any other path). */
 int sck_sweep(sck_handle *h, const sck_args *args, int32_t n, int32_t class_lo, int32_t class_hi, void *stream);
 
 /* sck_sweep of n whole sweeps with a HIP event pair on `stream` around every sweep-kernel launch (the class

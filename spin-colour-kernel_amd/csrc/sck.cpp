@@ -153,6 +153,8 @@ struct sck_handle {
     sck_launch resident{};
     bool has_tiled = false;
     sck_launch tiled{};
+    bool has_fold = false;
+    sck_launch fold{};        // running totals of E and M: adds one tiled sweep's partial sums
     size_t spin_bytes = 0;    // float32 spins of the whole batch (the tiled sweep's buffers)
     int sample_kernel = -1;
     int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
@@ -218,6 +220,10 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
     if (hi < 0 || hi > ns) hi = ns;
     if (lo < 0 || lo > hi) return fail(SCK_E_INVALID, "class range out of bounds");
     sck_args a = *args;
+    const bool tiled_path = h->has_tiled && !h->has_resident && lo == 0 && hi == ns && nb == 1 && args->dst && n > 0;
+    if (args->track && n > 0 && !(tiled_path && h->has_fold && args->track_parts))
+        return fail(SCK_E_INVALID, "running totals of E and M need whole tiled sweeps (a module with a tiled "
+                                   "kernel, args.dst and args.track_parts)");
     if (h->has_resident && lo == 0 && hi == ns && n > 0) {
         // whole sweeps: one launch, each workgroup runs its replica's n sweeps with the lattice in LDS
         a.sweep_offset = 0;
@@ -226,7 +232,7 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
         if (rc) return rc;
         return bump(h, *args, n, s);
     }
-    if (h->has_tiled && lo == 0 && hi == ns && nb == 1 && args->dst && n > 0) {
+    if (tiled_path) {
         // whole sweeps: one launch per sweep, every class of a (replica, tile) in LDS. A tile reads its halo
         // from other tiles' cells, so a sweep reads one buffer and writes the other (args->dst, a workspace of
         // the spins' size); after an odd number of sweeps the result is copied back into args->spins
@@ -236,6 +242,10 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
             a.dst = (i & 1) ? args->spins : args->dst;
             int rc = timed_launch(h, h->tiled, a, s, t);
             if (rc) return rc;
+            if (args->track) {
+                rc = launch(h, h->fold, a, s);
+                if (rc) return rc;
+            }
         }
         if (n & 1) HIP_TRY(hipMemcpyAsync(args->spins, args->dst, h->spin_bytes, hipMemcpyDeviceToDevice, s));
         return bump(h, *args, n, s);
@@ -344,6 +354,10 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
         }
         h->has_tiled = true;
         h->tiled = *d->tiled;
+        if (d->track_fold) {
+            h->has_fold = true;
+            h->fold = *d->track_fold;
+        }
         if (h->tiled.kernel < 0 || h->tiled.kernel >= (int)h->fn.size()) {
             sck_destroy(h);
             return fail(SCK_E_INVALID, "tiled launch names an unknown kernel");

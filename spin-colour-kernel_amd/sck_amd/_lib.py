@@ -22,7 +22,7 @@ LIB = PKG / "libsck.so"
 SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
-SCK_ABI_VERSION = 4
+SCK_ABI_VERSION = 5
 OBS_SCRATCH_BYTES = 32 << 20          # SCK_OBS_SCRATCH_BYTES of include/sck.h
 
 
@@ -32,7 +32,7 @@ class Args(C.Structure):
                 ("site_ku", C.c_void_p), ("site_axis", C.c_void_p), ("obs", C.c_void_p),
                 ("obs_scratch", C.c_void_p), ("sweep_offset", C.c_int64), ("nrep", C.c_int32),
                 ("nsweep", C.c_int32), ("aux", C.c_void_p), ("tiles", C.c_void_p), ("ntile", C.c_int32),
-                ("rep_base", C.c_int32)]
+                ("rep_base", C.c_int32), ("track", C.c_void_p), ("track_parts", C.c_void_p)]
 
 
 class LaunchC(C.Structure):
@@ -50,7 +50,7 @@ class Desc(C.Structure):
                 ("sync", C.POINTER(LaunchC)), ("n_sync", C.c_int32),
                 ("obs", C.POINTER(LaunchC)), ("n_obs", C.c_int32),
                 ("bump", C.POINTER(LaunchC)), ("resident", C.POINTER(LaunchC)), ("rep_blocks", C.c_int32),
-                ("pad1", C.c_int32), ("tiled", C.POINTER(LaunchC))]
+                ("pad1", C.c_int32), ("tiled", C.POINTER(LaunchC)), ("track_fold", C.POINTER(LaunchC))]
 
 
 #: every symbol include/sck.h declares

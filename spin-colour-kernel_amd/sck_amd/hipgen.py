@@ -96,6 +96,8 @@ class Module:
     resident: Launch | None = None     # whole sweeps of one replica per workgroup with the lattice in LDS
     tiled: Launch | None = None        # one whole sweep per launch, (replica, tile) per workgroup, tiles in LDS
     tile_dims: tuple | None = None     # the tiled kernel's cells per tile (x, y)
+    track_fold: Launch | None = None   # adds a tiled sweep's partial sums of E and M to the running totals
+    track_parts: int = 0               # doubles of those partial sums
     rep_blocks: int = 1                # the sweep list is this many groups of class launches, one per replica block
 
 
@@ -242,6 +244,7 @@ typedef struct sck_args {
     const int64_t *state; const float *width; const float *site_ku; const float *site_axis;
     double *obs; double *obs_scratch; int64_t sweep_offset; int32_t nrep; int32_t nsweep;
     const void *aux; const int32_t *tiles; int32_t ntile; int32_t rep_base;
+    double *track; double *track_parts;
 } sck_args;
 
 #define DEV static __device__ __forceinline__
@@ -1071,12 +1074,16 @@ class _Gen:
                 if bd.alpha == a:
                     offs[i] = (bd.alpha2 - a) * HW + bd.delta[1] * W + bd.delta[0]
             # beta and the cone width arrive per item (loaded once, ahead of the next tile's prefetch)
+            # the move's energy change and spin change are reported for the running totals of E and M
             w(f"DEV int tupd_{a}(const sck_args &A, float *__restrict__ ld, int li, int rep, int own, uint32_t key, "
-              f"float beta_r, float width_r) {{")
+              f"float beta_r, float width_r, float &dE_out, float *ds_out) {{")
             for k in range(n):
                 w(f"    float s{k} = ld[{k * NC} + li];")
             self.emit_local_field(a, nb=lambda i, k, offs=offs: f"ld[{k * NC} + li + ({offs[i]})]")
             self.emit_vector_tail(lambda k, v: f"ld[{k * NC} + li] = {v};", beta="beta_r", width="width_r")
+            w("    dE_out = dE;")
+            for k in range(n):
+                w(f"    ds_out[{k}] = t{k} - s{k};")
             w("    return acc;")
             w("}")
             w()
@@ -1101,6 +1108,9 @@ class _Gen:
         w(f'extern "C" __global__ void __launch_bounds__({RT}) __attribute__((amdgpu_waves_per_eu(1, {wpe}))) '
           f'sck_tiled(sck_args A) {{')
         w(f"    __shared__ __attribute__((aligned(16))) float ld[{b * HW * n}];")
+        NV = 1 + b * n
+        w(f"    __shared__ double red[{RT // 64}][{NV}];")
+        w("    const bool trk = A.track_parts != nullptr;      // running totals of E and M (uniform)")
         w(f"    const int64_t lo = (int64_t)blockIdx.x * {chunk};")
         w(f"    const int64_t hi = lo + {chunk} < {items}LL ? lo + {chunk} : {items}LL;")
         w("    const int64_t step = gridDim.y;")
@@ -1147,6 +1157,7 @@ class _Gen:
               "__builtin_bit_cast(int, A.width[rep])));")
         else:
             w("        const float width_r = 0.0f;")
+        w(f"        double tE = 0.0{''.join(f', tm{a}_{k} = 0.0' for a in range(b) for k in range(n))};")
         w("        __syncthreads();                          // the previous item's write-back has read the LDS")
         conds = [v for v in (vx, vy) if v != "true"]
         for u in range(ULD):
@@ -1215,10 +1226,32 @@ class _Gen:
             w(f"            const int own = {a * self.PL} + gy * {self.L1} + gx;")
             if self.has_mask:
                 w("            if (!A.mask[own]) continue;")
-            w(f"            tupd_{a}(A, ld, {a * HW} + ly * {W} + lx, rep, own, key, beta_r, width_r);")
+            w(f"            float dEo, dso[{n}];")
+            w(f"            const int acc = tupd_{a}(A, ld, {a * HW} + ly * {W} + lx, rep, own, key, beta_r, width_r, dEo, dso);")
+            # the running totals count each site once: its move in the tile itself (not the halo copies)
+            inner = [f"gxu < {self.L1}", f"gyu < {self.L2}"]
+            if e > 0:
+                inner = [f"lx >= {HX}", f"lx < {HX + TX}", f"ly >= {HY}", f"ly < {HY + TY}"] + inner
+            w(f"            if (trk && acc && {' && '.join(inner)}) {{")
+            w("                tE += (double)dEo;")
+            for k2 in range(n):
+                w(f"                tm{a}_{k2} += (double)dso[{k2}];")
+            w("            }")
             w("        }")
             w("        __syncthreads();")
         # write the tile back (its unwrapped cells inside the box: a partial tile's overhang is another tile's)
+        vals = ["tE"] + [f"tm{a}_{k}" for a in range(b) for k in range(n)]
+        w("        if (trk) {                                  // one partial of E and M per (replica, tile)")
+        for j, v in enumerate(vals):
+            w(f"            {{ double x = {v}; for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);"
+              f" if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][{j}] = x; }}")
+        w("            __syncthreads();")
+        w(f"            if (threadIdx.x < {NV}) {{")
+        w("                double x = 0.0;")
+        w(f"                for (int q = 0; q < {RT // 64}; ++q) x += red[q][threadIdx.x];")
+        w(f"                A.track_parts[((int64_t)tile * {self.B} + rep) * {NV} + threadIdx.x] = x;")
+        w("            }")
+        w("        }")
         w(f"        float *D = (float *)A.dst + (int64_t)rep * {n * self.NS};")
         w(f"        for (int i = threadIdx.x; i < {NST}; i += {RT}) {{")
         w(f"            const int a = i / {TY * NCI}, r = i - a * {TY * NCI};")
@@ -1233,6 +1266,18 @@ class _Gen:
             w(f"            __builtin_nontemporal_store(*(const {VT} *)(sl + {k * NC}), ({VT} *)(D + {self.NS * k} + q));")
         w("        }")
         w("    }")
+        w("}")
+        w()
+        # the running totals: one thread per (replica, value) adds that value's partials of every tile in tile
+        # order (a fixed order: the same totals run to run)
+        self.tiled_nt = NT
+        w('extern "C" __global__ void __launch_bounds__(256) sck_track_fold(sck_args A) {')
+        w("    const int i = blockIdx.x * blockDim.x + threadIdx.x;")
+        w(f"    if (i >= {self.B * NV}) return;")
+        w(f"    const int rep = i / {NV}, v = i - rep * {NV};")
+        w("    double x = 0.0;")
+        w(f"    for (int t = 0; t < {NT}; ++t) x += A.track_parts[((int64_t)t * {self.B} + rep) * {NV} + v];")
+        w(f"    if (v == 0) A.track[rep] += x; else A.track[{self.B} + (int64_t)rep * {NV - 1} + v - 1] += x;")
         w("}")
         w()
 
@@ -2025,7 +2070,7 @@ class _Gen:
         kernels = [f"sck_class_{k}" for k in range(len(self.classes))] + ["sck_seq"] + \
                   [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"] + \
                   (["sck_resident"] if rplan is not None else []) + \
-                  (["sck_tiled"] if tplan is not None else []) + \
+                  (["sck_tiled", "sck_track_fold"] if tplan is not None else []) + \
                   (["sck_sample"] if (self.n > 1 and self.mode == MODE_F32) else [])
         ki = {k: i for i, k in enumerate(kernels)}
         R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
@@ -2070,6 +2115,9 @@ class _Gen:
             # persistent: grid.y = 0 asks the launcher for the workgroups resident per XCD (from the occupancy)
             mod.tiled = Launch(ki["sck_tiled"], (XCDS, 0, 1), (self.TILE_THREADS, 1, 1))
             mod.tile_dims = tplan[:2]
+            nfold = self.B * (1 + self.b * self.n)
+            mod.track_fold = Launch(ki["sck_track_fold"], (-(-nfold // 256), 1, 1), (256, 1, 1))
+            mod.track_parts = self.tiled_nt * nfold     # doubles of one sweep's partial sums
         mod.key = hashlib.sha256(src.encode()).hexdigest()[:16]
         return mod
 

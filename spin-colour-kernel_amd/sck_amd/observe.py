@@ -21,23 +21,36 @@ import torch
 __all__ = ["series", "accumulate", "integrated_tau", "jackknife", "effective_samples", "cv_chi"]
 
 
-def series(sampler, S: torch.Tensor, n_meas: int, every: int = 1):
-    """(E (n_meas, B), |M|/N (n_meas, B), seconds) from n_meas replays of a graph of `every` sweeps followed by
-    one observation; E in float64 totals, |M| the norm of the total magnetization per active site."""
+def series(sampler, S: torch.Tensor, n_meas: int, every: int = 1, fused: bool = False, anchor_every: int = 100):
+    """(E (n_meas, B), |M|/N (n_meas, B), seconds) after each of n_meas replays of a graph of `every` sweeps;
+    E in float64 totals, |M| the norm of the total magnetization per active site.
+
+    fused=False: the graph ends with one observation (the float64 reductions over the lattice). fused=True (tiled
+    samplers): the sweeps keep running totals up to date (Sampler.track), re-anchored by an observation every
+    `anchor_every` measurements, so a measurement costs no pass over the lattice."""
     B = sampler.B
     N = float(sampler.sys.n_active)
     dev = sampler.device
-    rep = sampler.graph(S, every, observe=True)
+    if fused:
+        sampler.track(S)
+        rep = sampler.graph(S, every)
+        obs = sampler._track
+    else:
+        rep = sampler.graph(S, every, observe=True)
+        obs = sampler._obs
     nv = sampler.sys.b * sampler.sys.n
     out = torch.zeros((n_meas, B * (1 + nv)), dtype=torch.float64, device=dev)
-    obs = sampler._obs
     rep()                                      # warm-up replay
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     for i in range(n_meas):
+        if fused and i and i % max(1, int(anchor_every)) == 0:
+            sampler.track(S)
         rep()
         out[i].copy_(obs)
     torch.cuda.synchronize(dev)
+    if fused:
+        sampler.untrack()
     dt = time.perf_counter() - t0
     E = out[:, :B]
     M = out[:, B:].reshape(n_meas, B, sampler.sys.b, sampler.sys.n).sum(dim=2)

@@ -261,8 +261,10 @@ class Sampler:
             d.resident = C.cast(rarr, C.POINTER(L.LaunchC))
         if mod.tiled is not None:
             tarr, _ = L.launches([mod.tiled])
-            self._keep.append(tarr)
+            farr, _ = L.launches([mod.track_fold])
+            self._keep += [tarr, farr]
             d.tiled = C.cast(tarr, C.POINTER(L.LaunchC))
+            d.track_fold = C.cast(farr, C.POINTER(L.LaunchC))
         h = C.c_void_p()
         cache = str(cache_dir or L.CACHE).encode()
         with torch.cuda.device(self.device):
@@ -287,12 +289,16 @@ class Sampler:
         return self._ws
 
     def _args(self, S: torch.Tensor, dst: torch.Tensor | None = None, sweeps: bool = False) -> L.Args:
-        """sck_args over S; `sweeps`: for whole coloured sweeps (the tiled sweep's workspace as dst)."""
+        """sck_args over S; `sweeps`: for whole coloured sweeps (the tiled sweep's workspace as dst, and the
+        running totals of E and M when they are being tracked)."""
         a = L.Args()
         a.spins = S.data_ptr()
         if dst is None and sweeps and self.use_tiled:
             dst = self._workspace()
         a.dst = dst.data_ptr() if dst is not None else None
+        if sweeps and getattr(self, "_tracking", False):
+            a.track = self._track.data_ptr()
+            a.track_parts = self._track_parts.data_ptr()
         a.mask = self.mask.data_ptr()
         a.k1 = self.k1.data_ptr()
         a.beta = self.beta.data_ptr()
@@ -519,6 +525,34 @@ class Sampler:
             L.check(L.lib().sck_observe(self._handle, C.byref(a), C.c_void_p(_stream_ptr(self.device))), "observe")
         B, b, n = self.B, self.sys.b, self.sys.n
         return self._obs[:B].clone(), self._obs[B:].reshape(B, b, n).clone()
+
+    def track(self, S: torch.Tensor):
+        """Start (or re-anchor) float64 running totals of the energy and the per-sublattice magnetization: they
+        are set from one observation of S now, and every later whole sweep of the tiled kernel (sweep(), graph
+        replays) adds the energy and spin changes of its accepted moves, accumulated per (replica, tile) in the
+        sweep kernel itself. Between re-anchors they differ from observe() only by the fp32 rounding of each
+        move's dE. Needs the tiled sweep (2-D vector lattices in the reference layout); returns tracked()."""
+        if not self.use_tiled or self.module.track_fold is None:
+            raise ValueError("running totals of E and M need the tiled sweep (2-D vector lattices)")
+        if getattr(self, "_track", None) is None:
+            nv = 1 + self.sys.b * self.sys.n
+            self._track = torch.zeros(self.B * nv, dtype=torch.float64, device=self.device)
+            self._track_parts = torch.zeros(self.module.track_parts, dtype=torch.float64, device=self.device)
+        self.observe(S)
+        self._track.copy_(self._obs)
+        self._tracking = True
+        return self.tracked()
+
+    def tracked(self):
+        """Views (E (B,), M (B, b, n)) of the running totals started by track()."""
+        if not getattr(self, "_tracking", False):
+            raise ValueError("no running totals: call track(S) first")
+        B, b, n = self.B, self.sys.b, self.sys.n
+        return self._track[:B], self._track[B:].reshape(B, b, n)
+
+    def untrack(self) -> None:
+        """Stop updating the running totals (sweeps no longer accumulate them)."""
+        self._tracking = False
 
     def observed(self):
         """Views (E (B,), M (B, b, n)) of the float64 observation buffers: the last observe(), or the
