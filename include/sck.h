@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SCK_ABI_VERSION 5
+#define SCK_ABI_VERSION 6
 
 enum sck_status {
     SCK_OK = 0,
@@ -86,7 +86,8 @@ typedef struct sck_desc {
     int32_t n, b, L1, L2, L3; /* model and box */
     int32_t batch;            /* B, replicas */
     int32_t words;            /* W = ceil(B/32) for packed/bitwise, else B */
-    int32_t obs_chunks;       /* observable partial sums per replica */
+    int32_t obs_chunks;       /* observable partial sums per replica; args.obs_scratch holds
+                                 batch / max(1, obs_blocks) x obs_chunks x (1 + b n) doubles */
     const char *const *kernel_names; int32_t n_kernels;
     const sck_launch *sweep;  int32_t n_sweep;   /* one coloured sweep: class launches in sweep order */
     const sck_launch *ref;    int32_t n_ref;     /* one sequential sweep (reference chain)           */
@@ -99,7 +100,11 @@ typedef struct sck_desc {
                                  launches, one group per replica block. sck_sweep runs each block through
                                  all n sweeps before the next (replicas are independent chains, so the chain
                                  of every replica is unchanged) and a block's lattice stays cache-resident */
-    int32_t pad1;
+    int32_t obs_blocks;       /* 0 or 1: the observation list covers every replica at once. > 1 (equal to
+                                 rep_blocks): it is obs_blocks equal groups, one per replica block, each
+                                 writing its partial sums to the same first batch / obs_blocks rows of
+                                 args.obs_scratch; a graph captured with an observation observes each block
+                                 right after that block's sweeps */
     const sck_launch *tiled;                     /* NULL, or one whole sweep in one launch: each workgroup runs
                                                     every class of one (replica, tile) in LDS, reading
                                                     args.spins and writing args.dst. Whole sweeps use it when
@@ -197,12 +202,14 @@ int sck_observe(sck_handle *h, const sck_args *args, void *stream);
  * SCK_OBS_SCRATCH_BYTES in the library (args->obs_scratch is ignored), so the call allocates nothing (no hipMalloc,
  * hipFree or allocating library routine). No counterpart in the reference; it is for callers that cannot
  * allocate: code under HIP graph capture, tight serving loops, harnesses that forbid allocation. Returns
- * SCK_E_INVALID when the partial sums (batch x obs_chunks x (1 + b n) doubles) exceed that size. Every call on
+ * SCK_E_INVALID when the partial sums (batch / obs_blocks x obs_chunks x (1 + b n) doubles) exceed that size. Every call on
  * one GPU shares its array, which keeps the last call's contents; calls that overlap on one GPU race. */
 int sck_observe_noalloc(sck_handle *h, const sck_args *args, void *stream);
 
 /* Capture k sweeps (launches and counter bump; with observe != 0 an observation after them) in a HIP graph
- * (SCK/sampler.py:242-269). The graph reads the buffers of `args` by address. */
+ * (SCK/sampler.py:242-269). The graph reads the buffers of `args` by address. With replica blocks and a
+ * per-block observation list, each block's observation follows that block's k sweeps (the same values as
+ * observing after all of them: the blocks are independent chains). */
 int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t observe, sck_graph **out);
 int sck_graph_replay(sck_graph *g, void *stream);
 int sck_graph_destroy(sck_graph *g);

@@ -159,6 +159,7 @@ struct sck_handle {
     int sample_kernel = -1;
     int mode = 0, n = 0, b = 0, batch = 0, words = 0, obs_chunks = 1;
     int rep_blocks = 1;         // the sweep list holds rep_blocks groups of class launches, one per replica block
+    int obs_blocks = 1;         // the observation list holds obs_blocks groups (1, or rep_blocks: one per block)
 };
 
 struct sck_graph {
@@ -271,6 +272,34 @@ int do_observe(sck_handle *h, const sck_args *args, hipStream_t s) {
     return SCK_OK;
 }
 
+// k whole sweeps, then an observation. With one observation group per replica block, each block's group runs
+// right after that block's k sweeps, while the block's lattice is still in the cache; the blocks are
+// independent chains, so the values equal those of observing after every block has swept.
+int do_sweeps_observed(sck_handle *h, const sck_args *args, int32_t k, hipStream_t s) {
+    const int nb = h->rep_blocks;
+    if (h->obs_blocks <= 1 || h->obs_blocks != nb || h->has_resident || h->has_tiled || k <= 0) {
+        int rc = do_sweeps(h, args, k, 0, -1, s);
+        return rc ? rc : do_observe(h, args, s);
+    }
+    if (args->track) return fail(SCK_E_INVALID, "running totals of E and M need whole tiled sweeps");
+    const int ns = (int)h->sweep.size() / nb, no = (int)h->obs.size() / nb;
+    sck_args a = *args;
+    for (int rb = 0; rb < nb; ++rb) {
+        for (int32_t i = 0; i < k; ++i) {
+            a.sweep_offset = i;
+            for (int c = 0; c < ns; ++c) {
+                int rc = launch(h, h->sweep[(size_t)rb * ns + c], a, s);
+                if (rc) return rc;
+            }
+        }
+        for (int j = 0; j < no; ++j) {
+            int rc = launch(h, h->obs[(size_t)rb * no + j], *args, s);
+            if (rc) return rc;
+        }
+    }
+    return bump(h, *args, k, s);
+}
+
 // The partial sums of sck_observe_noalloc: a file-scope device array, one copy per GPU, which holds its
 // contents between calls (every call on a GPU shares it; calls that overlap on one GPU race). The host-side name is
 // only a handle: its device address is read once per device with hipGetSymbolAddress and used from then on.
@@ -317,6 +346,9 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
         return fail(SCK_E_INVALID, "descriptor launch plans missing");
     if (d->rep_blocks < 1 || d->n_sweep % d->rep_blocks != 0)
         return fail(SCK_E_INVALID, "rep_blocks must be >= 1 and divide the sweep launches");
+    const int obs_blocks = d->obs_blocks > 1 ? d->obs_blocks : 1;
+    if (obs_blocks > 1 && (obs_blocks != d->rep_blocks || d->n_obs % obs_blocks != 0 || d->batch % obs_blocks != 0))
+        return fail(SCK_E_INVALID, "obs_blocks must equal rep_blocks and divide the observation launches and the batch");
     std::string code;
     int rc = code_object(hip_src, cache_dir, code);
     if (rc) return rc;
@@ -387,6 +419,7 @@ int sck_create(const sck_desc *d, const char *hip_src, const char *cache_dir, sc
     h->words = d->words;
     h->obs_chunks = d->obs_chunks;
     h->rep_blocks = d->rep_blocks;
+    h->obs_blocks = obs_blocks;
     *out = h;
     return SCK_OK;
 }
@@ -459,7 +492,8 @@ int sck_observe(sck_handle *h, const sck_args *args, void *stream) {
 
 int sck_observe_noalloc(sck_handle *h, const sck_args *args, void *stream) {
     if (!h || !args || !args->obs) return fail(SCK_E_INVALID, "observe needs the obs output buffer");
-    const size_t need = (size_t)h->batch * (size_t)h->obs_chunks * (size_t)(1 + h->b * h->n) * sizeof(double);
+    const size_t rows = (size_t)(h->batch / h->obs_blocks);       // every observation group reuses these rows
+    const size_t need = rows * (size_t)h->obs_chunks * (size_t)(1 + h->b * h->n) * sizeof(double);
     if (need > SCK_OBS_SCRATCH_BYTES)
         return fail(SCK_E_INVALID, "observe_noalloc: the partial sums need " + std::to_string(need) +
                                        " bytes, more than the library's static scratch of " +
@@ -486,8 +520,7 @@ int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t ob
         delete g;
         return fail(SCK_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
     }
-    int rc = do_sweeps(h, args, k, 0, -1, g->stream);
-    if (!rc && observe) rc = do_observe(h, args, g->stream);
+    int rc = observe ? do_sweeps_observed(h, args, k, g->stream) : do_sweeps(h, args, k, 0, -1, g->stream);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(g->stream, &graph);
     if (rc || e2 != hipSuccess) {

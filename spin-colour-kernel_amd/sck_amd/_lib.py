@@ -22,7 +22,7 @@ LIB = PKG / "libsck.so"
 SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
-SCK_ABI_VERSION = 5
+SCK_ABI_VERSION = 6
 OBS_SCRATCH_BYTES = 32 << 20          # SCK_OBS_SCRATCH_BYTES of include/sck.h
 
 
@@ -50,7 +50,7 @@ class Desc(C.Structure):
                 ("sync", C.POINTER(LaunchC)), ("n_sync", C.c_int32),
                 ("obs", C.POINTER(LaunchC)), ("n_obs", C.c_int32),
                 ("bump", C.POINTER(LaunchC)), ("resident", C.POINTER(LaunchC)), ("rep_blocks", C.c_int32),
-                ("pad1", C.c_int32), ("tiled", C.POINTER(LaunchC)), ("track_fold", C.POINTER(LaunchC))]
+                ("obs_blocks", C.c_int32), ("tiled", C.POINTER(LaunchC)), ("track_fold", C.POINTER(LaunchC))]
 
 
 #: every symbol include/sck.h declares

@@ -325,6 +325,7 @@ class BatchSampler:
         d.L1 = d.L2 = d.L3 = 0
         d.batch, d.words, d.obs_chunks = self.R, self.R, 1
         d.rep_blocks = 1
+        d.obs_blocks = 1
         d.kernel_names = C.cast(names, C.POINTER(C.c_char_p))
         d.n_kernels = len(mod.kernels)
         for field_, plan in (("sweep", sweep), ("ref", ref), ("sync", []), ("obs", obs)):

@@ -99,6 +99,8 @@ class Module:
     track_fold: Launch | None = None   # adds a tiled sweep's partial sums of E and M to the running totals
     track_parts: int = 0               # doubles of those partial sums
     rep_blocks: int = 1                # the sweep list is this many groups of class launches, one per replica block
+    obs_blocks: int = 1                # the observation list is this many (partial sums, fold) groups, one per block
+    obs_rows: int = 0                  # replicas whose partial sums the observation scratch holds at once
 
 
 def _hexf(v: float) -> str:
@@ -1906,8 +1908,14 @@ class _Gen:
     # --------------------------------------------------------------------------------- observables
     def emit_observe(self, chunk: int) -> None:
         """Float64 energy and per-sublattice magnetization (SCK/sampler.py:296-345): each block sums the
-        sites [blockIdx.x * chunk, ...) of replica blockIdx.y into one partial; sck_obs_final adds the
-        partials of a replica in a fixed order."""
+        sites [blockIdx.x * chunk, ...) of its replicas into one partial per replica; sck_obs_final adds the
+        partials of a replica in a fixed order.
+
+        Replica-major: a workgroup is 64 replicas (the lanes) and 4 waves, each wave a quarter of the chunk's
+        sites; the waves' sums are added in LDS in wave order. A launch covers the replicas [A.rep_base,
+        A.rep_base + obs_span) and its partial sums sit at rows rep - A.rep_base of A.obs_scratch, so the
+        observation of a replica-blocked module runs block by block (each block's observation right after its
+        sweeps, while its lattice is cache-resident) and reuses one block's rows of scratch."""
         w, s, n, b = self.w, self.s, self.n, self.b
         packed = self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP)
         NV = 1 + b * n
@@ -1926,14 +1934,18 @@ class _Gen:
         w()
         w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_obs(sck_args A) {{')
         if self.replica_major:
-            # lanes over replicas: each thread sums its replica over the chunk's sites (uniform site addresses)
-            w("    const int rep = blockIdx.y * blockDim.x + threadIdx.x;")
-            w(f"    if (rep >= {self.B}) return;")
+            # lanes over replicas (uniform site addresses, one 256-byte access per spin component); the 4 waves
+            # split the chunk's sites
+            sub = -(-chunk // (BLOCK // 64))
+            w("    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;")
+            w("    const int rel = blockIdx.y * 64 + lane;")
+            w("    const int rep = A.rep_base + rel;")
+            w(f"    const bool live = rel < {self.obs_span} && rep < {self.B};")
             w(f"    double acc[{NV}];")
             w(f"    for (int k = 0; k < {NV}; ++k) acc[k] = 0.0;")
-            w(f"    const int lo = blockIdx.x * {chunk};")
-            w(f"    const int hi = min(lo + {chunk}, {self.NS});")
-            w("    for (int site = lo; site < hi; ++site) {")
+            w(f"    const int lo = blockIdx.x * {chunk} + wave * {sub};")
+            w(f"    const int hi = min(min(lo + {sub}, (int)(blockIdx.x + 1) * {chunk}), {self.NS});")
+            w("    for (int site = lo; live && site < hi; ++site) {")
         else:
             w("    const int rep = blockIdx.y;")
             w(f"    double acc[{NV}];")
@@ -1944,11 +1956,43 @@ class _Gen:
         self.emit_site_energy(lambda k, q: f"spin_at(A, rep, {k}, {q})")
         w("    }")
         if self.replica_major:
-            w(f"    for (int k = 0; k < {NV}; ++k) A.obs_scratch[((int64_t)rep * gridDim.x + blockIdx.x) * {NV} + k] = acc[k];")
+            W = BLOCK // 64
+            w(f"    __shared__ double red[{W}][{NV}][64];")
+            w(f"    for (int k = 0; k < {NV}; ++k) red[wave][k][lane] = acc[k];")
+            w("    __syncthreads();")
+            w("    if (wave == 0 && live) {")
+            w(f"        for (int k = 0; k < {NV}; ++k) {{")
+            w("            double v = red[0][k][lane];")
+            w(f"            for (int i = 1; i < {W}; ++i) v += red[i][k][lane];")
+            w(f"            A.obs_scratch[((int64_t)rel * gridDim.x + blockIdx.x) * {NV} + k] = v;")
+            w("        }")
+            w("    }")
             w("}")
         else:
             self.emit_obs_block_reduce(NV)
         self.emit_obs_final(NV)
+
+    def energy_bonds(self) -> dict:
+        """Bonds the observables sum, with their weight. The reference halves the sum over every directed bond
+        (SCK/sampler.py:326-341). Where every bond (a -> a2, delta) has its reverse (a2 -> a, -delta) with the
+        transposed matrix, summing one bond of each pair at weight 1 gives the same energy (the reverse bond's
+        term at a site is the forward bond's term at its neighbour, and a box edge or a mask drops both), with
+        half the neighbour reads; otherwise every bond is summed at weight 1/2."""
+        s = self.s
+        bonds = s.crystal.bonds
+        index = {(bd.alpha, bd.alpha2, tuple(bd.delta)): i for i, bd in enumerate(bonds)}
+        if len(index) != len(bonds):
+            return {i: 0.5 for i in range(len(bonds))}
+        keep = {}
+        paired = set()
+        for i, bd in enumerate(bonds):
+            j = index.get((bd.alpha2, bd.alpha, tuple(-d for d in bd.delta)))
+            if j is None or j == i or not np.array_equal(s.bond_matrix(j), s.bond_matrix(i).T):
+                return {i: 0.5 for i in range(len(bonds))}
+            if i not in paired:
+                keep[i] = 1.0
+                paired.add(j)
+        return keep
 
     def emit_site_energy(self, spin) -> None:
         """Loop body of the observables: site `site`'s coordinates, its spins s<k>, and its share of the energy
@@ -1961,24 +2005,26 @@ class _Gen:
         for k in range(n):
             w(f"        const double s{k} = {spin(k, 'site')};")
         w("        double e = 0.0;")
+        weights = self.energy_bonds()
         for a in range(b):
             w(f"        if (a == {a}) {{")
             for k in range(n):
                 w(f"            acc[{1 + a * n + k}] += s{k};")
             for i, bd in enumerate(s.crystal.bonds):
-                if bd.alpha != a:
+                if bd.alpha != a or i not in weights:
                     continue
                 M = s.bond_matrix(i)
                 if not M.any():
                     continue
                 q, ok = self.nb_index(i)
+                half = "0.5 * " if weights[i] == 0.5 else ""
                 w(f"            if ({ok}) {{ const int q = {q};")
                 for l in range(n):
                     w(f"                const double n{l} = {spin(l, 'q')};")
                 for k in range(n):
                     terms = [f"{_f64(M[k, l])} * n{l}" for l in range(n) if M[k, l] != 0.0]
                     if terms:
-                        w(f"                e -= 0.5 * s{k} * ({' + '.join(terms)});")
+                        w(f"                e -= {half}s{k} * ({' + '.join(terms)});")
                 w("            }")
             w("        }")
         if s.h is not None:
@@ -2017,17 +2063,37 @@ class _Gen:
     def emit_obs_final(self, NV: int) -> None:
         w = self.w
         w()
-        w(f'extern "C" __global__ void __launch_bounds__(64) sck_obs_final(sck_args A) {{')
-        w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
-        w(f"    if (rep >= {self.B}) return;")
-        w(f"    const int nch = {self.obs_chunks};")
-        w(f"    for (int k = 0; k < {NV}; ++k) {{")
-        w("        double v = 0.0;")
-        w(f"        for (int c = 0; c < nch; ++c) v += A.obs_scratch[((int64_t)rep * nch + c) * {NV} + k];")
-        w("        if (k == 0) A.obs[rep] = v;")
-        w(f"        else A.obs[{self.B} + (int64_t)rep * {NV - 1} + (k - 1)] = v;")
-        w("    }")
-        w("}")
+        if self.replica_major:
+            # a wave per replica: lane l adds partials l, l + 64, ... in order, then a butterfly over the lanes
+            # (the same association every run, so the result is deterministic)
+            w(f'extern "C" __global__ void __launch_bounds__({BLOCK}) sck_obs_final(sck_args A) {{')
+            w("    const int lane = threadIdx.x & 63;")
+            w(f"    const int rel = blockIdx.x * {BLOCK // 64} + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);")
+            w("    const int rep = A.rep_base + rel;")
+            w(f"    if (rel >= {self.obs_span} || rep >= {self.B}) return;")
+            w(f"    const int nch = {self.obs_chunks};")
+            w(f"    for (int k = 0; k < {NV}; ++k) {{")
+            w("        double v = 0.0;")
+            w(f"        for (int c = lane; c < nch; c += 64) v += A.obs_scratch[((int64_t)rel * nch + c) * {NV} + k];")
+            w("        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);")
+            w("        if (lane == 0) {")
+            w("            if (k == 0) A.obs[rep] = v;")
+            w(f"            else A.obs[{self.B} + (int64_t)rep * {NV - 1} + (k - 1)] = v;")
+            w("        }")
+            w("    }")
+            w("}")
+        else:
+            w(f'extern "C" __global__ void __launch_bounds__(64) sck_obs_final(sck_args A) {{')
+            w("    const int rep = blockIdx.x * blockDim.x + threadIdx.x;")
+            w(f"    if (rep >= {self.B}) return;")
+            w(f"    const int nch = {self.obs_chunks};")
+            w(f"    for (int k = 0; k < {NV}; ++k) {{")
+            w("        double v = 0.0;")
+            w(f"        for (int c = 0; c < nch; ++c) v += A.obs_scratch[((int64_t)rep * nch + c) * {NV} + k];")
+            w("        if (k == 0) A.obs[rep] = v;")
+            w(f"        else A.obs[{self.B} + (int64_t)rep * {NV - 1} + (k - 1)] = v;")
+            w("    }")
+            w("}")
         w()
         w('extern "C" __global__ void sck_bump(sck_args A) {')
         w("    if (threadIdx.x == 0 && blockIdx.x == 0) ((int64_t *)A.state)[1] += A.sweep_offset;")
@@ -2057,8 +2123,12 @@ class _Gen:
         else:
             self.emit_bitwise()
         # observables: enough blocks per replica for a full chip when replicas are few
+        self.obs_span = self.B
         if self.replica_major:
-            per = max(1, min(self.NS, -(-2048 // -(-self.B // BLOCK))))
+            # one launch per replica block: about 2048 workgroups of 64 replicas x 4 waves each
+            self.obs_span = self.rm_block
+            groups = -(-self.obs_span // 64)
+            per = max(1, min(-(-self.NS // (BLOCK // 64)), -(-2048 // groups)))
             chunk = -(-self.NS // per)
         else:
             per = max(1, min(64, (4096 + self.B - 1) // max(1, self.B)))
@@ -2100,15 +2170,24 @@ class _Gen:
                 sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.B // BLOCK), gy2, -(-self.PL // gy2))))
             else:
                 sync.append(Launch(ki[f"sck_sync_{a}"], (-(-self.PL // BLOCK), gy, gz)))
-        obs = [Launch(ki["sck_obs"], (self.obs_chunks, -(-self.B // BLOCK) if self.replica_major else self.B, 1)),
-               Launch(ki["sck_obs_final"], (-(-self.B // 64), 1, 1), (64, 1, 1))]
+        if self.replica_major:
+            # one (partial sums, fold) pair per replica block, each block's sums in the same scratch rows
+            span = self.obs_span
+            obs = []
+            for rb in range(nblk):
+                obs.append(Launch(ki["sck_obs"], (self.obs_chunks, -(-span // 64), 1), rep_base=rb * span))
+                obs.append(Launch(ki["sck_obs_final"], (-(-span // (BLOCK // 64)), 1, 1), rep_base=rb * span))
+        else:
+            obs = [Launch(ki["sck_obs"], (self.obs_chunks, self.B, 1)),
+                   Launch(ki["sck_obs_final"], (-(-self.B // 64), 1, 1), (64, 1, 1))]
         bump = Launch(ki["sck_bump"], (1, 1, 1), (64, 1, 1))
         layouts, offsets, total = ising_layout(s) if s.n == 1 else ([], [], 0)
         if nblk > 1:
             # block-major: the class launches of replica block 0, then of block 1, ...
             sweep = [replace(l, rep_base=rb * self.rm_block) for rb in range(nblk) for l in sweep]
         mod = Module(src, kernels, sweep, ref, sync, obs, bump, self.obs_chunks, self.classes,
-                     table_entries=total, ising_layout=layouts, rep_blocks=nblk)
+                     table_entries=total, ising_layout=layouts, rep_blocks=nblk,
+                     obs_blocks=nblk if self.replica_major else 1, obs_rows=self.obs_span)
         if rplan is not None:
             mod.resident = Launch(ki["sck_resident"], (self.B, 1, 1), (self.RESIDENT_THREADS, 1, 1))
         if tplan is not None:

@@ -67,7 +67,7 @@ class MI355XKernel:
         self.width = torch.zeros(1, dtype=torch.float32, device=dev)
         nv = 1 + s.b * s.n
         self.obs = torch.zeros(self.B * nv, dtype=torch.float64, device=dev)
-        self.obs_scratch = torch.zeros(self.B * self.mod.obs_chunks * nv, dtype=torch.float64, device=dev)
+        self.obs_scratch = torch.zeros(self.mod.obs_rows * self.mod.obs_chunks * nv, dtype=torch.float64, device=dev)
         self.table = None
         if s.n == 1 and not sampler.bitwise:
             self.table = torch.zeros((int(sampler.Bp), self.mod.table_entries), dtype=torch.float32, device=dev)
@@ -85,6 +85,7 @@ class MI355XKernel:
         d.L1, d.L2, d.L3 = s.L1, s.L2, s.L3
         d.batch, d.words, d.obs_chunks = self.B, self.W, mod.obs_chunks
         d.rep_blocks = mod.rep_blocks
+        d.obs_blocks = mod.obs_blocks
         d.kernel_names, d.n_kernels = C.cast(names, C.POINTER(C.c_char_p)), len(mod.kernels)
         for field, plan in (("sweep", mod.sweep), ("ref", mod.ref), ("sync", mod.sync), ("obs", mod.obs)):
             arr, n = _lib.launches(plan)

@@ -230,7 +230,8 @@ class Sampler:
         self._mask_f = torch.as_tensor(m, device=dev).to(torch.float32)
         nv = 1 + b * n
         self._obs = torch.zeros(self.B * nv, dtype=torch.float64, device=dev)
-        self._obs_scratch = torch.zeros(self.B * self.module.obs_chunks * nv, dtype=torch.float64, device=dev)
+        self._obs_scratch = torch.zeros(self.module.obs_rows * self.module.obs_chunks * nv, dtype=torch.float64,
+                                        device=dev)
         self._handle = None
         self._create(cache_dir)
 
@@ -245,6 +246,7 @@ class Sampler:
         d.n, d.b, d.L1, d.L2, d.L3 = self.sys.n, self.sys.b, self.sys.L1, self.sys.L2, self.sys.L3
         d.batch, d.words, d.obs_chunks = self.B, self.W, mod.obs_chunks
         d.rep_blocks = mod.rep_blocks
+        d.obs_blocks = mod.obs_blocks
         d.kernel_names = C.cast(names, C.POINTER(C.c_char_p))
         d.n_kernels = len(mod.kernels)
         for field_, plan in (("sweep", mod.sweep), ("ref", mod.ref), ("sync", mod.sync), ("obs", mod.obs)):

@@ -127,7 +127,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_create_rejects_incomplete_descriptors_before_any_device_work():
     """sck_create checks the descriptor before it compiles or loads anything: a missing counter-bump plan, a
-    plan count without its array, or a replica-block count that does not divide the sweep launches."""
+    plan count without its array, a replica-block count that does not divide the sweep launches, or per-block
+    observation groups that do not match the replica blocks."""
     from sck_amd import _lib as L
     lib = L.lib()
     names = (ctypes.c_char_p * 1)(b"sck_class_0")
@@ -148,7 +149,8 @@ def test_create_rejects_incomplete_descriptors_before_any_device_work():
 
     h = ctypes.c_void_p()
     for d, msg in [(desc(bump=None), b"launch plans missing"), (desc(n_ref=2), b"launch plans missing"),
-                   (desc(rep_blocks=2), b"rep_blocks"), (desc(abi_version=L.SCK_ABI_VERSION + 1), b"ABI")]:
+                   (desc(rep_blocks=2), b"rep_blocks"), (desc(abi_version=L.SCK_ABI_VERSION + 1), b"ABI"),
+                   (desc(obs_blocks=2, obs=ctypes.cast(plan, ctypes.POINTER(L.LaunchC)), n_obs=2, batch=2), b"obs_blocks")]:
         assert lib.sck_create(ctypes.byref(d), b"// unused", None, ctypes.byref(h)) != 0
         assert msg in lib.sck_last_error()
         assert not h.value

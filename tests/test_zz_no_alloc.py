@@ -34,7 +34,7 @@ def _case(s, B, **kw):
     A.set_temperature(torch.linspace(0.5, 2.5, B))
     S = A.init_spins(seed=2)
     A.sweep(S, 3)
-    nbytes = A.B * A.module.obs_chunks * (1 + s.b * s.n) * 8
+    nbytes = A.module.obs_rows * A.module.obs_chunks * (1 + s.b * s.n) * 8
     E0, M0 = A.observe(S)                                   # the entry point with a caller workspace
     E1, M1 = _noalloc(A, S)
     assert torch.equal(E0, E1) and torch.equal(M0, M1)
@@ -49,10 +49,12 @@ def test_noalloc_observe_equals_observe_on_two_calls():
 
 
 def test_noalloc_observe_refuses_more_than_its_array():
-    # Heisenberg fcc (4 sublattices): 65536 replicas x 8 partial blocks x 13 sums x 8 bytes = 54.5 MB
+    # Heisenberg fcc (4 sublattices), one unblocked replica-major launch: 393216 replicas x 1 partial x 13 sums
+    # x 8 bytes = 40.9 MB
     s = sy.heisenberg("fcc", 2)
-    A = Sampler(s, 65536, DEV, seed=1, layout="replica")
-    need = A.B * A.module.obs_chunks * (1 + s.b * s.n) * 8
+    A = Sampler(s, 393216, DEV, seed=1, layout="replica")
+    assert A.module.obs_rows == A.B
+    need = A.module.obs_rows * A.module.obs_chunks * (1 + s.b * s.n) * 8
     assert need > L.OBS_SCRATCH_BYTES
     S = A.init_spins(seed=1)
     with pytest.raises(ValueError, match="static scratch"):
