@@ -272,13 +272,14 @@ DEV void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32
     }
 }
 
-// philox10 for the counter (c0, 0, 0, 0) and key (k0, 0), the form every sweep draw uses. Round 1 has one
-// product (the other multiplies c2 = 0) and leaves c0 = k0, c1 = 0, so round 2's second product depends on the
-// key alone and is uniform across a wave when the key is; rounds 3-10 are the general ones.
+// philox10 for the counter (c0, c1, 0, 0) and key (k0, 0), the form every sweep draw uses (c1 = 0 unless the
+// counter is 64-bit). Round 1 has one product (the other multiplies c2 = 0) and leaves c0 = c1 ^ k0, c1 = 0, so
+// round 2's second product depends on c1 and the key alone and is uniform across a wave when they are; rounds
+// 3-10 are the general ones.
 DEV void philox10_z(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0) {
     const uint64_t q = (uint64_t)0xD2511F53u * (uint64_t)c0;
     const uint64_t p0 = (uint64_t)0xCD9E8D57u * (uint64_t)(uint32_t)(q >> 32);
-    const uint64_t pk = (uint64_t)0xD2511F53u * (uint64_t)k0;
+    const uint64_t pk = (uint64_t)0xD2511F53u * (uint64_t)(c1 ^ k0);
     c0 = (uint32_t)(p0 >> 32) ^ (k0 + 0x9E3779B9u);
     c1 = (uint32_t)p0;
     c2 = (uint32_t)q ^ ((uint32_t)(pk >> 32) ^ 0xBB67AE85u);
@@ -722,17 +723,21 @@ class _Gen:
 
     def emit_draw(self, ind: str) -> None:
         w = self.w
-        if self.wide:
+        if self.wide and not self.rt and self.NS & (self.NS - 1) == 0:
+            # NS = 2^k: rep * NS has k low zero bits and own < NS, so no carry reaches the high word, which is
+            # rep >> (32 - k): uniform wherever the replica is (the tiled kernel), so Philox's first rounds run
+            # partly on the scalar unit
+            k = self.NS.bit_length() - 1
+            w(f"{ind}uint32_t c0 = (uint32_t)rep * {self.NS}u + (uint32_t)own, c1 = (uint32_t)rep >> {32 - k}, "
+              f"c2 = 0u, c3 = 0u;")
+        elif self.wide:
             ns = f"{self.NS}ull" if not self.rt else f"(uint64_t){self.NS}"
             w(f"{ind}const uint64_t ctr = (uint64_t)rep * {ns} + (uint64_t)own;")
             w(f"{ind}uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = 0u, c3 = 0u;")
         else:
             ns = f"{self.NS}u" if not self.rt else f"(uint32_t){self.NS}"
             w(f"{ind}uint32_t c0 = (uint32_t)rep * {ns} + (uint32_t)own, c1 = 0u, c2 = 0u, c3 = 0u;")
-        if self.wide:
-            w(f"{ind}philox10(c0, c1, c2, c3, key, 0u);")
-        else:
-            w(f"{ind}philox10_z(c0, c1, c2, c3, key);")
+        w(f"{ind}philox10_z(c0, c1, c2, c3, key);")
         w(f"{ind}const float u0 = u01(c0);")
         if self.n > 1:
             w(f"{ind}const float u1 = u01(c1);")

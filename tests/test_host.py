@@ -289,9 +289,9 @@ def test_sampler_argument_refusals(kw, msg):
 
 
 def test_specialised_philox_equals_general_and_oracle(tmp_path):
-    """philox10_z (counter (c0, 0, 0, 0), key (k0, 0); rounds 1-2 simplified) equals the general philox10 of
+    """philox10_z (counter (c0, c1, 0, 0), key (k0, 0); rounds 1-2 simplified) equals the general philox10 of
     the same generated header and the oracle's Philox-4x32-10, compiled for the host (bitop3 as its truth
-    table)."""
+    table), for c1 = 0 (32-bit counters) and for random high words (64-bit counters)."""
     import subprocess
     import oracle as O
     src = hg.HEADER
@@ -299,6 +299,8 @@ def test_specialised_philox_equals_general_and_oracle(tmp_path):
     rng = np.random.default_rng(5)
     c0 = np.concatenate([[0, 1, 0xFFFFFFFF], rng.integers(0, 2 ** 32, 200)]).astype(np.uint32)
     k0 = np.concatenate([[0, 0xFFFFFFFF, 7], rng.integers(0, 2 ** 31, 200)]).astype(np.uint32)
+    c0, k0 = np.concatenate([c0, c0]), np.concatenate([k0, k0])
+    c1 = np.concatenate([np.zeros(203, np.uint32), [1, 0xFFFFFFFF, 3], rng.integers(0, 2 ** 32, 200)]).astype(np.uint32)
     prog = ["#include <stdint.h>", "#include <stdio.h>", "#define DEV static inline",
             "static uint32_t __builtin_amdgcn_bitop3_b32(uint32_t a, uint32_t b, uint32_t c, unsigned t) {",
             "    uint32_t r = 0;",
@@ -310,8 +312,9 @@ def test_specialised_philox_equals_general_and_oracle(tmp_path):
             "}", body, "int main(void) {",
             f"    static const uint32_t C0[{len(c0)}] = {{{', '.join(f'{v}u' for v in c0)}}};",
             f"    static const uint32_t K0[{len(k0)}] = {{{', '.join(f'{v}u' for v in k0)}}};",
+            f"    static const uint32_t C1[{len(c1)}] = {{{', '.join(f'{v}u' for v in c1)}}};",
             f"    for (int i = 0; i < {len(c0)}; ++i) {{",
-            "        uint32_t a0 = C0[i], a1 = 0, a2 = 0, a3 = 0, b0 = C0[i], b1, b2, b3;",
+            "        uint32_t a0 = C0[i], a1 = C1[i], a2 = 0, a3 = 0, b0 = C0[i], b1 = C1[i], b2, b3;",
             "        philox10(a0, a1, a2, a3, K0[i], 0u);",
             "        philox10_z(b0, b1, b2, b3, K0[i]);",
             '        printf("%u %u %u %u %u %u %u %u\\n", a0, a1, a2, a3, b0, b1, b2, b3);',
@@ -325,7 +328,7 @@ def test_specialised_philox_equals_general_and_oracle(tmp_path):
                     if l.strip()], dtype=np.uint64)
     assert np.array_equal(out[:, :4], out[:, 4:])
     for i in range(len(c0)):
-        ref = O.philox((int(k0[i]), 0), [[int(c0[i]), 0, 0, 0]])[0]
+        ref = O.philox((int(k0[i]), 0), [[int(c0[i]), int(c1[i]), 0, 0]])[0]
         assert np.array_equal(out[i, :4], ref.astype(np.uint64)), i
 
 
