@@ -628,6 +628,8 @@ def ess_summary(log=None) -> dict:
             "gain_M": r["gain_M"], "fixed_width": r["fixed_width"],
             "observables_frac_within_2sigma": r["frac_within_2sigma"], "observables_max_z": r["max_z"],
             "ess_E_per_s": {k: r[k]["ess_E_per_s"] for k in ("uniform", "fixed", "learned")},
+            "loop_updates_per_s": {k: r[k]["updates_per_s"] for k in ("uniform", "fixed", "learned")},
+            "table_vs_fine_tau_E": r.get("table_vs_fine_tau_E"),
             "train_seconds": r["train_seconds"]}
 
 

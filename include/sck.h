@@ -44,7 +44,9 @@ enum sck_mode {
  * state_ptr, x_only, row0, ...) (SCK/generator.py:191-194; SCK/bitwise.py:278-281). */
 typedef struct sck_args {
     void *spins;              /* S [B][n][b][L3][L2][L1] (or words [W][1][b][L3][L2][L1]); updated in place */
-    void *dst;                /* synchronous control only: destination of the update            */
+    void *dst;                /* synchronous control: destination of the update. Whole sweeps of a module
+                                 with a tiled launch: a workspace of the spins' size (the sweeps alternate
+                                 between the two buffers; the result ends in spins). Else NULL          */
     const int8_t *mask;       /* [b][L3][L2][L1], 1 = spin present                               */
     const float *k1;          /* [b][L3][L2][L1] cubic K1 per site                               */
     const float *beta;        /* [Bp] inverse temperature per replica                            */
@@ -54,7 +56,7 @@ typedef struct sck_args {
     const float *site_ku;     /* [b][L3][L2][L1] per-site uniaxial strength                      */
     const float *site_axis;   /* [b][L3][L2][L1][n] per-site uniaxial axis                        */
     double *obs;              /* observables: output E [B] then M [B][b][n]; partial sums in obs_scratch */
-    double *obs_scratch;      /* [B][chunks][1 + b*n] per-block partial sums                      */
+    double *obs_scratch;      /* [B / obs_blocks][chunks][1 + b*n] per-block partial sums          */
     int64_t sweep_offset;     /* sweep index = state[1] + sweep_offset                            */
     int32_t nrep;             /* replicas B (words W for packed/bitwise)                          */
     int32_t nsweep;           /* sweeps one launch of a resident kernel performs (set by the library) */

@@ -34,8 +34,8 @@ from .sampler import Sampler
 KB = 0.086173404     # meV / K
 PARAM_NAMES = ("Jex2", "Jex3", "Jex4", "Kan1", "KanS", "Hex", "KDM")
 
-__all__ = ["KB", "PARAM_NAMES", "build_state", "active_shells", "check_exact", "Observer", "anneal_state",
-           "reduce_obs", "main"]
+__all__ = ["KB", "PARAM_NAMES", "ACC_ROWS", "build_state", "active_shells", "check_exact", "Observer",
+           "anneal_state", "reduce_obs", "main"]
 
 
 def build_state(params, Rd: float, L: int, angle_deg: float, shells=(1,)) -> sy.System:
@@ -158,26 +158,60 @@ def anneal_state(s: sy.System, replicas: int, temps_K, n_therm: int, n_meas: int
                 flips_per_ns=B * observe.n_act * sweeps / dt / 1e9)
 
 
+# the rows of the accumulator array (n_T, 8, B), in the file format's order
+ACC_ROWS = ("dE", "dE2", "dM", "dM2", "Mz", "Q", "E_ref", "M_ref")
+
+
 def reduce_obs(obs, temps, n_spins, kB: float = KB) -> dict:
-    """Per-temperature scalars from the accumulators (n_T, 8, B) (SCK/anneal.py:221-234): E per spin and
-    |M| (means and spreads over replicas), Mz, Cv = var(E) / (kB T^2 N), chi = var(|M|) / (kB T) and Q,
-    each variance taken per replica about its reference and then averaged over replicas."""
-    obs = np.asarray(obs, dtype=np.float64)
-    if obs.ndim != 3 or obs.shape[1] != 8:
-        raise ValueError(f"expected accumulators of shape (n_T, 8, batch), got {obs.shape}")
-    dE, dE2, dM, dM2, Mz, Q, E_ref, M_ref = (obs[:, i] for i in range(8))
-    T = np.asarray(temps, dtype=np.float64)[:, None]
-    E = E_ref + dE
-    M = M_ref + dM
-    Cv = (dE2 - dE * dE) / (kB * T * T * n_spins)
-    chi = (dM2 - dM * dM) / (kB * T)
-    return dict(T=np.asarray(temps), E=E.mean(1) / n_spins, E_sd=E.std(1) / n_spins, M=M.mean(1), M_sd=M.std(1),
-                Mz=Mz.mean(1), Cv=Cv.mean(1), chi=chi.mean(1), Q=Q.mean(1), Q_sd=Q.std(1))
+    """Per-temperature scalars from the accumulators (SCK/anneal.py:221-234): E per spin and |M| (means over
+    replicas, with their spreads), Mz, Q (with its spread), Cv = var(E) / (kB T^2 N) and chi = var(|M|) / (kB T).
+    Each replica's variance is taken from its own offsets about its reference value, then averaged."""
+    acc = np.asarray(obs, dtype=np.float64)
+    if acc.ndim != 3 or acc.shape[1] != len(ACC_ROWS):
+        raise ValueError(f"expected accumulators of shape (n_T, {len(ACC_ROWS)}, batch), got {acc.shape}")
+    row = dict(zip(ACC_ROWS, acc.transpose(1, 0, 2)))
+    T = np.asarray(temps, dtype=np.float64)
+    kT = kB * T[:, None]
+
+    def spread(first, second):
+        """Variance from the first two moments of the offsets."""
+        return row[second] - row[first] ** 2
+
+    per_replica = {
+        "E": (row["E_ref"] + row["dE"]) / n_spins,
+        "M": row["M_ref"] + row["dM"],
+        "Mz": row["Mz"],
+        "Cv": spread("dE", "dE2") / (kT * T[:, None] * n_spins),
+        "chi": spread("dM", "dM2") / kT,
+        "Q": row["Q"],
+    }
+    out = {"T": np.asarray(temps)}
+    for key, values in per_replica.items():
+        out[key] = values.mean(axis=1)
+        if key in ("E", "M", "Q"):
+            out[key + "_sd"] = values.std(axis=1)
+    return out
+
+
+def _ladder(cfg: dict):
+    """The temperatures of a run configuration and its split of each temperature's sweeps: two thirds
+    thermalize, the rest measure."""
+    temps = np.linspace(float(cfg["T_start"]), float(cfg["T_end"]), int(cfg["n_temps"]))
+    total = int(cfg["sweeps"])
+    return temps, total * 2 // 3, total - total * 2 // 3
+
+
+def _selected(spec: str, count: int) -> list[int]:
+    chosen = [int(v) for v in spec.split(",") if v.strip()]
+    bad = [i for i in chosen if not 0 <= i < count]
+    if bad:
+        raise SystemExit(f"state indices must lie in [0, {count}); got {bad}")
+    return chosen
 
 
 def main(argv=None) -> None:
-    """Process states of a run configuration on one GPU, one state_<name>.npz each (SCK/anneal.py:237-266).
-    Config keys: params (rows), names, T_start, T_end, n_temps, sweeps (two thirds thermalize), Rd, layers,
+    """Process states of a run configuration on one GPU and write one state_<name>.npz each
+    (SCK/anneal.py:237-266). Config keys: params (rows), names, T_start, T_end, n_temps, sweeps, Rd, layers,
     angle, replicas, every, seed."""
     ap = argparse.ArgumentParser(prog="python -m sck_amd.anneal")
     ap.add_argument("--config", type=Path, required=True)
@@ -187,29 +221,29 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     cfg = json.loads(a.config.read_text())
     rows = cfg["params"]
-    idx = [int(v) for v in a.states.split(",") if v.strip()]
-    if any(i < 0 or i >= len(rows) for i in idx):
-        raise SystemExit(f"state indices must lie in [0, {len(rows)})")
+    chosen = _selected(a.states, len(rows))
     a.out.mkdir(parents=True, exist_ok=True)
+    temps, n_therm, n_meas = _ladder(cfg)
     shells = active_shells(rows)
-    temps = np.linspace(float(cfg["T_start"]), float(cfg["T_end"]), int(cfg["n_temps"]))
-    n_therm = int(cfg["sweeps"]) * 2 // 3
-    n_meas = int(cfg["sweeps"]) - n_therm
-    for i in idx:
+    for i in chosen:
         name = cfg["names"][i]
+
+        def say(msg: str, name=name) -> None:
+            print(f"[{name}] {msg.strip()}", flush=True)
+
         s = build_state(rows[i], cfg["Rd"], cfg["layers"], cfg["angle"], shells)
-        d = check_exact(s, a.device)
-        print(f"[{name}] {s.colouring.n_colours} colours ({s.colouring.describe()}), "
-              f"exactness against the sequential chain: {d} differing spins", flush=True)
-        if d:
-            raise SystemExit(f"[{name}] the coloured sweep does not reproduce the sequential chain")
+        differ = check_exact(s, a.device)
+        say(f"{s.colouring.n_colours} colours ({s.colouring.describe()}); coloured sweeps against the sequential "
+            f"chain: {differ} spins differ")
+        if differ:
+            raise SystemExit(f"state {name}: the coloured sweep does not reproduce the sequential chain")
         r = anneal_state(s, int(cfg["replicas"]), temps, n_therm, n_meas, int(cfg["every"]), int(cfg["seed"]) + i,
-                         a.device, log=lambda msg: print(f"[{name}]" + msg, flush=True))
+                         a.device, log=say)
         np.savez_compressed(a.out / f"state_{name}.npz", params=np.asarray(rows[i], np.float32), obs=r["obs"],
                             M=np.transpose(r["mvec"], (1, 0, 2)), snaps=r["snaps"],
                             temperatures=temps.astype(np.float32), n_active=s.n_active, seconds=r["seconds"],
                             flips_per_ns=r["flips_per_ns"], colours=s.colouring.n_colours)
-        print(f"[{name}] done in {r['seconds']:.1f} s, {r['flips_per_ns']:.2f} spin flips/ns", flush=True)
+        say(f"finished in {r['seconds']:.1f} s at {r['flips_per_ns']:.2f} spin flips/ns")
 
 
 if __name__ == "__main__":

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from .observe import cv_chi, effective_samples, jackknife, series
-from .proposal import best_fixed_width, train_width
+from .proposal import LearnedWidth, best_fixed_width, train_width
 from .sampler import Sampler
 
 __all__ = ["compare_proposals"]
@@ -66,11 +66,13 @@ def _zscores(a: dict, b: dict) -> np.ndarray:
 
 def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 1000, therm: int = 50,
                       n_meas: int = 1000, every: int = 1, seed: int = 0, train_steps: int = 400, log=print,
-                      widths=(0.5, 1.0, 1.5, 2.0, 2.5), init=None) -> dict:
+                      widths=(0.5, 1.0, 1.5, 2.0, 2.5), init=None, fine_table: bool = True) -> dict:
     """Train the width on `warmup` fixed-width sweeps, equilibrate one state for `equil` sweeps with the
     learned width, then from that common equilibrium state run `therm` + `n_meas` sweeps of each of: the
     uniform redraw (the reference's proposal), each fixed width of `widths`, and the learned width. The
-    fixed baseline is the width with the largest geometric-mean ESS/s of E over the ladder, measured."""
+    fixed baseline is the width with the largest geometric-mean ESS/s of E over the ladder, measured. With
+    fine_table, the learned network also runs from a table 4x finer along every feature: the geometric mean over
+    the ladder of tau_E(table) / tau_E(fine table) is the table's cost in effective samples per sweep."""
     temps = np.asarray(temps, np.float32)
     per = B // len(temps)
     T_rep = np.repeat(temps, per)
@@ -96,9 +98,14 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     Q.set_temperature(torch.as_tensor(T_rep))
     state = S0.clone()
     Q.sweep(state, equil)
+    # the same network tabulated 4x finer along every feature: what the sweep's table costs against the
+    # network it stands for
+    lw_fine = LearnedWidth(lw.params, dict(lw.cfg), n=tuple(4 * int(x) for x in lw.n),
+                           s=tuple(4.0 * float(x) for x in lw.s))
     kinds = [("uniform", dict(proposal="uniform"))] + \
         [(f"fixed_{w:g}", dict(proposal="cone", width=float(w))) for w in widths] + \
-        [("learned", dict(proposal="learned", learned=lw))]
+        [("learned", dict(proposal="learned", learned=lw))] + \
+        ([("learned_fine", dict(proposal="learned", learned=lw_fine))] if fine_table else [])
     for label, kw in kinds:
         A = Sampler(system, B, "cuda", seed=seed + 1, **kw)
         A.set_temperature(torch.as_tensor(T_rep))
@@ -135,6 +142,9 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     result["gain_E_per_T_best_over_fixed"] = float(np.exp(np.mean(np.log(
         ceil * np.array([result["fixed"]["tau_E"][j] * result["fixed"]["seconds"] for j in range(nt)])))))
     result["gain_E_vs_per_T_best"] = float(np.exp(np.mean(np.log(lrn / ceil))))
+    if fine_table:
+        result["table_vs_fine_tau_E"] = float(np.exp(np.mean(np.log(
+            np.array(result["learned"]["tau_E"], float) / np.array(result["learned_fine"]["tau_E"], float)))))
     result["vs_uniform"] = _gains(result["uniform"], result["learned"], nt)
     result["fixed_vs_uniform"] = _gains(result["uniform"], result["fixed"], nt)
     # agreement of the observables: learned against each baseline
