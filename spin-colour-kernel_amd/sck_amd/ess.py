@@ -143,8 +143,8 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
         ceil * np.array([result["fixed"]["tau_E"][j] * result["fixed"]["seconds"] for j in range(nt)])))))
     result["gain_E_vs_per_T_best"] = float(np.exp(np.mean(np.log(lrn / ceil))))
     if fine_table:
-        result["table_vs_fine_tau_E"] = float(np.exp(np.mean(np.log(
-            np.array(result["learned"]["tau_E"], float) / np.array(result["learned_fine"]["tau_E"], float)))))
+        result["table_vs_fine_tau_E"] = float(np.exp(np.mean([
+            np.log(result["learned"]["tau_E"][j] / result["learned_fine"]["tau_E"][j]) for j in range(nt)])))
     result["vs_uniform"] = _gains(result["uniform"], result["learned"], nt)
     result["fixed_vs_uniform"] = _gains(result["uniform"], result["fixed"], nt)
     # agreement of the observables: learned against each baseline
