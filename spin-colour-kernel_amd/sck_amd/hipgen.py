@@ -662,7 +662,11 @@ class _Gen:
         `rbase`, in the reference's fp32 order: J groups in first-appearance order (v = v + J g), then the
         extra matrices of the extra bonds in bond order, then the field (SCK/generator.py:222-277).
         nb(i, k), when given, is the expression of component k of bond i's neighbour (always present: a tile in
-        LDS holds zeros outside an open box)."""
+        LDS holds zeros outside an open box).
+
+        A sum starts from its first term rather than from 0.0f + that term (which the compiler may not fold): the
+        two differ only in the sign of a zero sum, which changes neither dE's test (exp(-beta * (+-0)) = 1) nor
+        any spin."""
         s, n, w = self.s, self.n, self.w
         groups, extras = _groups(s, a)
         for k in range(n):
@@ -670,15 +674,17 @@ class _Gen:
         for j in range(len(extras)):
             for k in range(n):
                 w(f"{ind}float n{j}_{k} = 0.0f;")
+        v_set = False
         for Jv, members in groups:
             w(f"{ind}{{")
             for k in range(n):
-                w(f"{ind}    float g{k} = 0.0f;")
-            for i in members:
+                w(f"{ind}    float g{k};")
+            for mi, i in enumerate(members):
+                add = (lambda k: f"g{k} = t{k};") if mi == 0 else (lambda k: f"g{k} = g{k} + t{k};")
                 if nb is not None:
                     w(f"{ind}    {{")
                     for k in range(n):
-                        w(f"{ind}      const float t{k} = {nb(i, k)}; g{k} = g{k} + t{k};")
+                        w(f"{ind}      const float t{k} = {nb(i, k)}; {add(k)}")
                         if i in extras:
                             w(f"{ind}      n{extras.index(i)}_{k} = t{k};")
                     w(f"{ind}    }}")
@@ -692,16 +698,20 @@ class _Gen:
                     w(f"{ind}    {{ const bool ok = {ok}; const int q = ok ? ({q}) : own;")
                 for k in range(n):
                     if ok == "true":
-                        w(f"{ind}      const float t{k} = (float){self.at('src', k, 'q')}; g{k} = g{k} + t{k};")
+                        w(f"{ind}      const float t{k} = (float){self.at('src', k, 'q')}; {add(k)}")
                     else:
                         w(f"{ind}      const float r{k} = (float){self.at('src', k, 'q')};")
-                        w(f"{ind}      const float t{k} = ok ? r{k} : 0.0f; g{k} = g{k} + t{k};")
+                        w(f"{ind}      const float t{k} = ok ? r{k} : 0.0f; {add(k)}")
                     if i in extras:
                         w(f"{ind}      n{extras.index(i)}_{k} = t{k};")
                 w(f"{ind}    }}")
             if Jv != 0.0:
                 for k in range(n):
-                    w(f"{ind}    v{k} = v{k} + {_hexf(Jv)} * g{k};")
+                    if v_set:
+                        w(f"{ind}    v{k} = v{k} + {_hexf(Jv)} * g{k};")
+                    else:
+                        w(f"{ind}    v{k} = {_hexf(Jv)} * g{k};")
+                v_set = True
             w(f"{ind}}}")
         for j, i in enumerate(extras):
             E = np.asarray(s.extra[i], np.float64)
@@ -1224,7 +1234,13 @@ class _Gen:
                 w(f"            const int m = u / {nv}, ri = u - m * {nv};")
                 w(f"            const int ly = {y_lo} + {Py} * m + {rsel}, q0 = {qsel};")
             w(f"            const int lx = {x_lo} + q0 + {P} * j;")
-            w(f"            if (ly >= {y_lo + hy} || lx >= {x_lo + wx}) continue;")
+            # a bound is tested only where the enumeration can pass it (the largest ly and lx it produces)
+            ly_max = y_lo + Py * (nblk - 1) + max(r for r, _ in res)
+            lx_max = x_lo + max(q for _, q in res) + P * (bxk - 1)
+            outside = ([f"ly >= {y_lo + hy}"] if ly_max >= y_lo + hy else []) + \
+                ([f"lx >= {x_lo + wx}"] if lx_max >= x_lo + wx else [])
+            if outside:
+                w(f"            if ({' || '.join(outside)}) continue;")
             w(f"            const int gxu = tx0 - {HX} + lx, gyu = ty0 - {HY} + ly;")
             w(f"            {cx} {cy}")
             conds = [v for v in (vx, vy) if v != "true"]
