@@ -23,7 +23,7 @@ from .observe import cv_chi, effective_samples, jackknife, series
 from .proposal import LearnedWidth, best_fixed_width, train_width
 from .sampler import Sampler
 
-__all__ = ["compare_proposals"]
+__all__ = ["compare_proposals", "summarize_proposals"]
 
 
 def _stats(E, Ma, temps_rep, temps, N):
@@ -120,17 +120,27 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
                              observables=_stats(E, Ma, groups, temps, N))
         log(f"  {label:8s}: {secs:7.2f} s, ESS/s (E) {essE / secs:10.1f}, ESS/s (|M|) {essM / secs:10.1f}, "
             f"updates/s {B * N * n_meas * every / secs:.3e}")
-    nt = len(temps)
+    summarize_proposals(result, len(temps), widths, fine_table)
+    log(f"  best fixed width by measured ESS/s: {result['fixed_width']}")
+    return result
+
+
+def summarize_proposals(result: dict, nt: int, widths, fine_table: bool = True) -> dict:
+    """The comparison's figures from the per-proposal runs in `result` (keys "uniform", "fixed_<w>" for w in
+    widths, "learned" and, with fine_table, "learned_fine"; each with seconds, tau_E and tau_M per temperature
+    group, ess_E_per_s, ess_M_per_s and observables): the best fixed width by geometric-mean ESS/s of E, the
+    learned width's gains over it and over the uniform redraw, the per-temperature ceiling of constant widths,
+    the table's cost against the finer table, and the agreement of the observables. Updates `result`."""
     # the fixed baseline: the measured-best constant width (geometric mean over the ladder of ESS/s of E)
     def gm(r):
         return float(np.exp(np.mean([np.log(1.0 / (r["tau_E"][j] * r["seconds"])) for j in range(nt)])))
     fixed_labels = [f"fixed_{w:g}" for w in widths]
     best = max(fixed_labels, key=lambda k: gm(result[k]))
     result["fixed"] = result[best]
+    # the width the training objective picked, replaced by the one measured best
+    result["fixed_width_by_objective"] = result.get("fixed_width")
     result["fixed_width"] = float(best.split("_", 1)[1])
-    result["fixed_width_by_objective"] = w_fixed
     result["fixed_scan_gm_ess"] = {k: gm(result[k]) for k in fixed_labels + ["uniform", "learned"]}
-    log(f"  best fixed width by measured ESS/s: {result['fixed_width']}")
     result.update(_gains(result["fixed"], result["learned"], nt))
     # the ceiling of any width-only proposal that is constant per temperature: at every temperature the best
     # of the scanned widths and the uniform redraw, each by its own measured ESS/s of E
