@@ -88,6 +88,8 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     lw = train_width(W, S0, warmup=warmup, steps=train_steps, log=log)
     t_train = time.perf_counter() - t0
     w_fixed, scan = best_fixed_width(W, lw.samples)
+    del W                                      # its spins and workspace are no longer needed
+    torch.cuda.empty_cache()
     log(f"  fixed-width scan (width, mean log expected squared jump): "
         f"{[(w, round(j, 5)) for w, j in scan]} -> {w_fixed}")
     result = dict(system=system.name, B=B, temps=temps.tolist(), warmup=warmup, equil=equil, therm=therm,
@@ -96,8 +98,10 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
     # one equilibrium state for all proposals: any chain that keeps detailed balance leaves it stationary
     Q = Sampler(system, B, "cuda", seed=seed + 7, proposal="learned", learned=lw)
     Q.set_temperature(torch.as_tensor(T_rep))
-    state = S0.clone()
+    state, S0 = S0, None                       # the warm-up state itself: nothing else reads it
     Q.sweep(state, equil)
+    del Q
+    torch.cuda.empty_cache()
     # the same network tabulated 4x finer along every feature: what the sweep's table costs against the
     # network it stands for
     lw_fine = LearnedWidth(lw.params, dict(lw.cfg), n=tuple(4 * int(x) for x in lw.n),
@@ -111,10 +115,14 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
         A.set_temperature(torch.as_tensor(T_rep))
         S = state.clone()
         A.sweep(S, therm)
-        E, Ma, secs = series(A, S, n_meas, every)
+        # the tiled sweep keeps running totals of E and M, so its measurements need no pass over the lattice
+        fused = bool(getattr(A, "use_tiled", False)) and A.module.track_fold is not None
+        E, Ma, secs = series(A, S, n_meas, every, fused=fused)
         essE, tauE = effective_samples(E, groups)
         essM, tauM = effective_samples(Ma, groups)
-        result[label] = dict(seconds=secs, sweeps=n_meas * every, ess_E=essE, ess_M=essM,
+        del A, S
+        torch.cuda.empty_cache()
+        result[label] = dict(seconds=secs, sweeps=n_meas * every, fused=fused, ess_E=essE, ess_M=essM,
                              ess_E_per_s=essE / secs, ess_M_per_s=essM / secs, tau_E=tauE, tau_M=tauM,
                              updates_per_s=B * N * n_meas * every / secs,
                              observables=_stats(E, Ma, groups, temps, N))
