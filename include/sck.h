@@ -208,7 +208,15 @@ int sck_observe(sck_handle *h, const sck_args *args, void *stream);
  * one GPU shares its array, which keeps the last call's contents; calls that overlap on one GPU race. */
 int sck_observe_noalloc(sck_handle *h, const sck_args *args, void *stream);
 
-/* Capture k sweeps (launches and counter bump; with observe != 0 an observation after them) in a HIP graph
+/* Flags of sck_graph_capture's `observe` argument. */
+#define SCK_CAPTURE_OBSERVE 1   /* an observation after the sweeps */
+#define SCK_CAPTURE_IN_DST 2    /* whole tiled sweeps leave an odd count's result in args->dst instead of copying it
+                                   back into args->spins, and the observation reads it there: two graphs with the
+                                   buffers exchanged, replayed in turn, measure after every sweep with no copies.
+                                   No counterpart in the reference; other paths update args->spins in place as
+                                   always. */
+
+/* Capture k sweeps (launches and counter bump; with SCK_CAPTURE_OBSERVE an observation after them) in a HIP graph
  * (SCK/sampler.py:242-269). The graph reads the buffers of `args` by address. With replica blocks and a
  * per-block observation list, each block's observation follows that block's k sweeps (the same values as
  * observing after all of them: the blocks are independent chains). */

@@ -214,14 +214,21 @@ int timed_launch(sck_handle *h, const sck_launch &l, const sck_args &a, hipStrea
     return SCK_OK;
 }
 
+// Whether n sweeps over classes [lo, hi) run through the tiled kernel (whole sweeps with a workspace).
+bool tiled_sweeps(const sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_t hi) {
+    const int nb = h->rep_blocks, ns = (int)h->sweep.size() / nb;
+    if (hi < 0 || hi > ns) hi = ns;
+    return h->has_tiled && !h->has_resident && lo == 0 && hi == ns && nb == 1 && args->dst && n > 0;
+}
+
 int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_t hi, hipStream_t s,
-              launch_timer *t = nullptr) {
+              launch_timer *t = nullptr, bool copy_back = true) {
     // lo, hi: class range; the sweep list is rep_blocks groups of ns class launches
     const int nb = h->rep_blocks, ns = (int)h->sweep.size() / nb;
     if (hi < 0 || hi > ns) hi = ns;
     if (lo < 0 || lo > hi) return fail(SCK_E_INVALID, "class range out of bounds");
     sck_args a = *args;
-    const bool tiled_path = h->has_tiled && !h->has_resident && lo == 0 && hi == ns && nb == 1 && args->dst && n > 0;
+    const bool tiled_path = tiled_sweeps(h, args, n, lo, hi);
     if (args->track && n > 0 && !(tiled_path && h->has_fold && args->track_parts))
         return fail(SCK_E_INVALID, "running totals of E and M need whole tiled sweeps (a module with a tiled "
                                    "kernel, args.dst and args.track_parts)");
@@ -248,7 +255,9 @@ int do_sweeps(sck_handle *h, const sck_args *args, int32_t n, int32_t lo, int32_
                 if (rc) return rc;
             }
         }
-        if (n & 1) HIP_TRY(hipMemcpyAsync(args->spins, args->dst, h->spin_bytes, hipMemcpyDeviceToDevice, s));
+        // an odd count leaves the result in args->dst: copied back, unless the caller keeps it there
+        if ((n & 1) && copy_back)
+            HIP_TRY(hipMemcpyAsync(args->spins, args->dst, h->spin_bytes, hipMemcpyDeviceToDevice, s));
         return bump(h, *args, n, s);
     }
     // each replica block runs all n sweeps before the next one starts
@@ -520,7 +529,20 @@ int sck_graph_capture(sck_handle *h, const sck_args *args, int32_t k, int32_t ob
         delete g;
         return fail(SCK_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
     }
-    int rc = observe ? do_sweeps_observed(h, args, k, g->stream) : do_sweeps(h, args, k, 0, -1, g->stream);
+    int rc;
+    if ((observe & SCK_CAPTURE_OBSERVE) && h->obs_blocks > 1) {
+        rc = do_sweeps_observed(h, args, k, g->stream);
+    } else {
+        // SCK_CAPTURE_IN_DST: whole tiled sweeps leave an odd count's result in args->dst (no copy back) and
+        // the observation reads it there
+        const bool keep = (observe & SCK_CAPTURE_IN_DST) != 0;
+        rc = do_sweeps(h, args, k, 0, -1, g->stream, nullptr, !keep);
+        if (!rc && (observe & SCK_CAPTURE_OBSERVE)) {
+            sck_args o = *args;
+            if (keep && (k & 1) && tiled_sweeps(h, args, k, 0, -1)) o.spins = args->dst;
+            rc = do_observe(h, &o, g->stream);
+        }
+    }
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(g->stream, &graph);
     if (rc || e2 != hipSuccess) {

@@ -23,7 +23,9 @@ SOURCES = [ROOT / "csrc" / "sck.cpp", ROOT / "csrc" / "mlp.hip"]
 CACHE = Path(os.environ.get("SCK_AMD_CACHE", ROOT / "_kcache"))
 
 SCK_ABI_VERSION = 6
-OBS_SCRATCH_BYTES = 32 << 20          # SCK_OBS_SCRATCH_BYTES of include/sck.h
+OBS_SCRATCH_BYTES = 32 << 20
+CAPTURE_OBSERVE = 1        # sck_graph_capture flags (include/sck.h)
+CAPTURE_IN_DST = 2          # SCK_OBS_SCRATCH_BYTES of include/sck.h
 
 
 class Args(C.Structure):

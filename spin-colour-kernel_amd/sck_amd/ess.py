@@ -117,7 +117,7 @@ def compare_proposals(system, B: int, temps, warmup: int = 200, equil: int = 100
         A.sweep(S, therm)
         # the tiled sweep keeps running totals of E and M, so its measurements need no pass over the lattice
         fused = bool(getattr(A, "use_tiled", False)) and A.module.track_fold is not None
-        E, Ma, secs = series(A, S, n_meas, every, fused=fused)
+        E, Ma, secs = series(A, S, n_meas, every, fused=fused, log=log)
         essE, tauE = effective_samples(E, groups)
         essM, tauM = effective_samples(Ma, groups)
         del A, S

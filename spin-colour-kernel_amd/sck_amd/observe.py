@@ -21,33 +21,43 @@ import torch
 __all__ = ["series", "accumulate", "integrated_tau", "jackknife", "effective_samples", "cv_chi"]
 
 
-def series(sampler, S: torch.Tensor, n_meas: int, every: int = 1, fused: bool = False, anchor_every: int = 100):
+def series(sampler, S: torch.Tensor, n_meas: int, every: int = 1, fused: bool = False, anchor_every: int = 100,
+           log=None):
     """(E (n_meas, B), |M|/N (n_meas, B), seconds) after each of n_meas replays of a graph of `every` sweeps;
     E in float64 totals, |M| the norm of the total magnetization per active site.
 
     fused=False: the graph ends with one observation (the float64 reductions over the lattice). fused=True (tiled
     samplers): the sweeps keep running totals up to date (Sampler.track), re-anchored by an observation every
-    `anchor_every` measurements, so a measurement costs no pass over the lattice."""
+    `anchor_every` measurements, so a measurement costs no pass over the lattice. With the tiled sweep and an odd
+    `every`, two graphs with the spins and the workspace exchanged are replayed in turn (Sampler.graph_pair), so
+    no measurement copies the lattice back; S holds the final state on return. `log`, if given, receives a
+    progress line about every tenth of the measurements."""
     B = sampler.B
     N = float(sampler.sys.n_active)
     dev = sampler.device
     if fused:
         sampler.track(S)
-        rep = sampler.graph(S, every)
-        obs = sampler._track
-    else:
-        rep = sampler.graph(S, every, observe=True)
-        obs = sampler._obs
+    there, back, W = sampler.graph_pair(S, every, observe=not fused)
+    obs = sampler._track if fused else sampler._obs
     nv = sampler.sys.b * sampler.sys.n
     out = torch.zeros((n_meas, B * (1 + nv)), dtype=torch.float64, device=dev)
-    rep()                                      # warm-up replay
+    there()                                    # warm-up: both replays, the state back in S
+    back()
+    if fused:
+        sampler.track(S)
     torch.cuda.synchronize(dev)
+    step = max(1, n_meas // 10)
     t0 = time.perf_counter()
     for i in range(n_meas):
+        cur = W if i % 2 else S                # the buffer holding the state before this replay
         if fused and i and i % max(1, int(anchor_every)) == 0:
-            sampler.track(S)
-        rep()
+            sampler.track(cur)
+        (back if i % 2 else there)()
         out[i].copy_(obs)
+        if log is not None and (i + 1) % step == 0:
+            log(f"    {i + 1}/{n_meas} measurements, {time.perf_counter() - t0:.1f} s")
+    if n_meas % 2 and W is not S:
+        S.copy_(W)
     torch.cuda.synchronize(dev)
     if fused:
         sampler.untrack()

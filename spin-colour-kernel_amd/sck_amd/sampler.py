@@ -464,11 +464,31 @@ class Sampler:
         self._check_spins(S)
         if not self._is_native(S):
             raise ValueError("graph capture needs spins in the sampler's storage order (as init_spins returns them)")
-        a = self._args(S, sweeps=True)
+        return self._capture(S, None, k, L.CAPTURE_OBSERVE if observe else 0)
+
+    def graph_pair(self, S: torch.Tensor, k: int, observe: bool = False):
+        """Two replays for measuring after every k sweeps (k odd) with the tiled sweep and no copies: the first
+        sweeps S into the sampler's workspace (and observes it there), the second sweeps the workspace back
+        into S. Replayed in turn they run the chain graph() runs; after an odd number of replays the state is in
+        the workspace (returned third, with S's shape). For other kernels, or even k, both replays are graph()."""
+        self._check_spins(S)
+        if not self._is_native(S):
+            raise ValueError("graph capture needs spins in the sampler's storage order (as init_spins returns them)")
+        flags = L.CAPTURE_OBSERVE if observe else 0
+        if not (self.use_tiled and k % 2 == 1):
+            g = self._capture(S, None, k, flags)
+            return g, g, S
+        W = self._workspace().view_as(S)
+        there = self._capture(S, W, k, flags | L.CAPTURE_IN_DST)
+        back = self._capture(W, S, k, flags | L.CAPTURE_IN_DST)
+        return there, back, W
+
+    def _capture(self, S: torch.Tensor, dst, k: int, flags: int):
+        a = self._args(S, dst=dst, sweeps=True)
         g = C.c_void_p()
         with torch.cuda.device(self.device):
             torch.cuda.current_stream(self.device).synchronize()
-            L.check(L.lib().sck_graph_capture(self._handle, C.byref(a), int(k), int(bool(observe)), C.byref(g)),
+            L.check(L.lib().sck_graph_capture(self._handle, C.byref(a), int(k), int(flags), C.byref(g)),
                     "graph capture")
         dev = self.device
         lib = L.lib()
@@ -493,7 +513,7 @@ class Sampler:
                 except Exception:
                     pass
 
-        replay.keep = (self, S, _Keep(g))
+        replay.keep = (self, S, dst, _Keep(g))
         replay.time_ms = time_ms
         return replay
 

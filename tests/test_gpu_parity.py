@@ -394,6 +394,21 @@ def test_replica_blocks_equal_unblocked_sweeps(make, B, blk, chunks):
     Q.sweep(S2, 1)
     assert nl == len(P.classes) * (B // blk) and torch.equal(S1, S2)
     assert int(P.state[1]) == int(Q.state[1]) == 10
+    # an observing graph runs each block's observation right after that block's sweeps: every replay gives
+    # the spins of the unblocked sampler, and observables equal to an eager observation of the same state
+    # (and to the unblocked module's, up to float64 summation order)
+    assert P.module.obs_blocks == B // blk
+    obs = P.graph(S1, 2, observe=True)
+    for _ in range(3):
+        obs()
+        Q.sweep(S2, 2)
+        torch.cuda.synchronize()
+        assert torch.equal(S1, S2)
+        Eg, Mg = [x.clone() for x in P.observed()]
+        Ee, Me = P.observe(S1)
+        assert torch.equal(Eg, Ee) and torch.equal(Mg, Me)
+        Eq, Mq = Q.observe(S2)
+        assert torch.allclose(Ee, Eq, rtol=1e-12, atol=1e-9) and torch.allclose(Me, Mq, rtol=1e-12, atol=1e-9)
 
 
 @pytest.mark.parametrize("resident", [False, True])

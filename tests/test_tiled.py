@@ -148,3 +148,34 @@ def test_tiled_full_size_xy_equals_class_launches():
         assert torch.equal(S1[r0:r0 + 128], S2[r0:r0 + 128]), f"replicas {r0}..{r0 + 127} differ"
     changed = (S1[:64] != X0).any(dim=1).float().mean().item()
     assert 0.05 < changed < 0.95
+
+
+@pytest.mark.gpu
+def test_graph_pair_equals_graph_replays():
+    """Sampler.graph_pair: two graphs with the spins and the workspace exchanged (no copy back after the odd sweep
+    count), replayed in turn, run the chain that graph() replays run, and observe the same values after every
+    replay; after an odd number of replays the state is in the workspace."""
+    from sck_amd.sampler import Sampler
+    make, kw, dims = SYSTEMS["xy_triangular_98"]
+    s = make()
+    B = 16
+    print(f"[gpu] graph pair {s.name} {s.box}", flush=True)
+    A = Sampler(s, B, DEV, seed=5, tiled=True, tile_dims=dims, resident=False, **kw)
+    R = Sampler(s, B, DEV, seed=5, tiled=True, tile_dims=dims, resident=False, **kw)
+    for X in (A, R):
+        X.set_temperature(torch.linspace(0.4, 1.6, B))
+    S1 = A.init_spins(seed=3)
+    S2 = S1.clone()
+    there, back, W = A.graph_pair(S1, 1, observe=True)
+    assert W.data_ptr() != S1.data_ptr()
+    ref = R.graph(S2, 1, observe=True)
+    for i in range(5):
+        (back if i % 2 else there)()
+        ref()
+        torch.cuda.synchronize()
+        cur = S1 if i % 2 else W
+        assert torch.equal(cur, S2), i
+        Ea, Ma = A.observed()
+        Er, Mr = R.observed()
+        assert torch.equal(Ea, Er) and torch.equal(Ma, Mr), i
+    assert int(A.state[1]) == int(R.state[1]) == 5

@@ -61,3 +61,17 @@ def test_noalloc_observe_refuses_more_than_its_array():
         _noalloc(A, S)
     E, _ = A.observe(S)                                     # the workspace entry point still serves it
     assert torch.isfinite(E).all()
+
+
+def test_noalloc_observe_with_replica_blocks():
+    """A replica-blocked module (4 blocks of 64 replicas): every block's partial sums reuse the same rows of the
+    static array, and the result equals sck_observe's."""
+    print("[gpu] observe_noalloc: Heisenberg bcc 8^3 B=256 in 4 replica blocks", flush=True)
+    A = Sampler(sy.heisenberg("bcc", 8), 256, DEV, seed=4, layout="replica", replica_block=64)
+    assert A.module.obs_blocks == 4 and A.module.obs_rows == 64
+    A.set_temperature(torch.linspace(0.5, 2.5, 256))
+    S = A.init_spins(seed=2)
+    A.sweep(S, 3)
+    E0, M0 = A.observe(S)
+    E1, M1 = _noalloc(A, S)
+    assert torch.equal(E0, E1) and torch.equal(M0, M1)
