@@ -13,7 +13,9 @@ nanodisk's decisions match the oracle's; the sweep samples the single-spin distr
 """
 
 import itertools
+import json
 import math
+from dataclasses import asdict
 
 import numpy as np
 import pytest
@@ -359,6 +361,8 @@ def test_autotuned_batch_keeps_the_chain():
     S = S0.clone()
     Bt.sweep(S, 2)                                           # tunes on a copy first
     assert Bt.tuning and Bt.schedule is not None and int(Bt.state[1]) == 2
+    tuned = json.loads(Bt.schedule_json())                   # the chosen schedule and the timings behind it
+    assert tuned["schedule"] == asdict(Bt.schedule) and tuned["tuning"] == json.loads(json.dumps(Bt.tuning))
     ref = _individual(systems, [reps] * 6, [np.full(3, 1.1, np.float32)] * 6, Bt.views(S0), 2, 2)
     for i, v in enumerate(Bt.views(S)):
         assert torch.equal(v, ref[i]), i

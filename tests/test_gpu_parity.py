@@ -340,6 +340,27 @@ def test_observables_match_oracle():
         assert np.allclose(M.cpu().numpy(), Mo, rtol=1e-12, atol=1e-9)
 
 
+def test_set_seed_restarts_the_stream_in_place():
+    """set_seed(seed, sweep) rewrites the state pair the kernels read (SCK/sampler.py:148-150): the chain then
+    continues exactly as a Sampler built with that seed whose sweep counter stands at `sweep`, and a captured
+    graph sees the new seed, as it sees beta."""
+    s = sy.ising("square", 16)
+    A = Sampler(s, 64, DEV, seed=1, packed=False)
+    Bs = Sampler(s, 64, DEV, seed=9, packed=False)
+    for X in (A, Bs):
+        X.set_temperature(2.2)
+    S1 = A.init_spins(seed=4)
+    S2 = S1.clone()
+    rep = A.graph(S1, 2)
+    A.set_seed(9, 5)
+    Bs.state[1] = 5
+    rep()
+    Bs.sweep(S2, 2)
+    torch.cuda.synchronize()
+    assert torch.equal(S1, S2)
+    assert int(A.state[0]) == 9 and int(A.state[1]) == 7
+
+
 def test_graph_replay_equals_eager_sweeps_and_reads_beta_by_address():
     s = sy.heisenberg("bcc", 4)
     A, Bs = Sampler(s, 8, DEV, seed=2), Sampler(s, 8, DEV, seed=2)
