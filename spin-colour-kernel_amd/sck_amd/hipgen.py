@@ -786,10 +786,18 @@ class _Gen:
             if self.s.cubic_k1 is not None:
                 kexpr = f"({kexpr} + fabsf(A.k1[own]))"
             w(f"    const float f0 = hv * bt;")
-            w(f"    const float f1 = {kexpr} * bt;")
+            # a feature that is zero for every finite field is emitted as the constant (strict fp32 keeps the
+            # compiler from folding 0 * v, and f2 would cost a division): a model without anisotropy has f1 = 0,
+            # and one whose spins have no component along the axis (XY with the z axis) f2 = 0
+            zero1 = self.s.uniaxial is None and self.s.site_ku is None and self.s.cubic_k1 is None
+            zero2 = all(float(lm['axis'][k]) == 0.0 for k in range(n))
+            w("    const float f1 = 0.0f;" if zero1 else f"    const float f1 = {kexpr} * bt;")
             ve = self._lassoc([f"v{k} * {_hexf(lm['axis'][k])}" for k in range(n)])
-            w(f"    const float f2 = hv > 0.0f ? fabsf({ve}) / hv : 0.0f;")
+            w("    const float f2 = 0.0f;" if zero2 else f"    const float f2 = hv > 0.0f ? fabsf({ve}) / hv : 0.0f;")
             for d in range(3):
+                if (d == 1 and zero1) or (d == 2 and zero2):
+                    w(f"    const int i{d} = 0;")
+                    continue
                 w(f"    int i{d} = (int)(f{d} * {_hexf(lm['s'][d])}); i{d} = i{d} > {lm['n'][d] - 1} ? {lm['n'][d] - 1} : (i{d} < 0 ? 0 : i{d});")
             w(f"    const float wd = A.width[(i0 * {lm['n'][1]} + i1) * {lm['n'][2]} + i2];")
         if n == 2:
