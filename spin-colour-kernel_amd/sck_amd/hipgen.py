@@ -1409,6 +1409,22 @@ class _Gen:
                 w(f"                p_[u] = valid ? isp_{a}(lat, 0, own, x, y, z, tbl, &s0_[u]) : 0.0f;")
                 w("                own_[u] = own; valid_[u] = valid;")
                 w("            }")
+                # the draws of all the group's sites in one block: where a lane needs any of them, every site's
+                # Philox chain runs there (a wave almost always has lanes that need each site's draw, so this
+                # costs no instructions), and the independent chains interleave instead of running one branch
+                # after the other
+                w(f"            float ud_[{nb}];")
+                w("            bool any = false;")
+                w("#pragma unroll")
+                w(f"            for (int u = 0; u < {nb}; ++u) any = any || (valid_[u] && !{self.decided_expr('p_[u]')});")
+                w("            if (any) {")
+                w("#pragma unroll")
+                w(f"                for (int u = 0; u < {nb}; ++u) {{")
+                w("                    const int own = own_[u];")
+                self.emit_draw("                    ")
+                w("                    ud_[u] = u0;")
+                w("                }")
+                w("            }")
                 w("#pragma unroll")
                 w(f"            for (int u = 0; u < {nb}; ++u) {{")
                 w("                const int own = own_[u], s0 = s0_[u];")
@@ -1416,7 +1432,7 @@ class _Gen:
                 w(f"                if (valid_[u] && {self.decided_expr('p')}) {{")
                 w(f"                    if ({self.decided_value('p')}) lat[own] = ({T})(-s0);")
                 w("                } else if (valid_[u]) {")
-                self.emit_draw("                    ")
+                w("                    const float u0 = ud_[u];")
                 w(f"                    if ({self.accept_expr('p')}) lat[own] = ({T})(-s0);")
                 w("                }")
                 w("            }")
