@@ -1094,7 +1094,13 @@ class _Gen:
         NCI = TX // V if V > 1 else TX
         NST = b * TY * NCI                  # chunks of the tile itself
         VT = "f32x4" if V == 4 else "float"
-        NC = b * HW                         # LDS cells per component: the tile is component-planar
+        NC = b * HW                         # LDS cells per component
+        # XY tiles whose rows move as float4 chunks keep a cell's two components side by side in LDS, so each
+        # neighbour read is one 8-byte access at an immediate offset; other tiles are component-planar
+        IL = n == 2 and V == 4
+
+        def lds(cell: str, k: int) -> str:
+            return f"ld[2 * ({cell}) + {k}]" if IL else f"ld[{k * NC} + {cell}]"
         for a in range(b):
             offs = {}
             for i, bd in enumerate(s.crystal.bonds):
@@ -1105,9 +1111,9 @@ class _Gen:
             w(f"DEV int tupd_{a}(const sck_args &A, float *__restrict__ ld, int li, int rep, int own, uint32_t key, "
               f"float beta_r, float width_r, float &dE_out, float *ds_out) {{")
             for k in range(n):
-                w(f"    float s{k} = ld[{k * NC} + li];")
-            self.emit_local_field(a, nb=lambda i, k, offs=offs: f"ld[{k * NC} + li + ({offs[i]})]")
-            self.emit_vector_tail(lambda k, v: f"ld[{k * NC} + li] = {v};", beta="beta_r", width="width_r")
+                w(f"    float s{k} = {lds('li', k)};")
+            self.emit_local_field(a, nb=lambda i, k, offs=offs: lds(f"li + ({offs[i]})", k))
+            self.emit_vector_tail(lambda k, v: f"{lds('li', k)} = {v};", beta="beta_r", width="width_r")
             w("    dE_out = dE;")
             for k in range(n):
                 w(f"    ds_out[{k}] = t{k} - s{k};")
@@ -1195,14 +1201,20 @@ class _Gen:
                 w(f"            if (i < {NLD}) {{")
             w(f"            const int a = i / {H * NCH}, r = i - a * {H * NCH};")
             w(f"            const int ly = r / {NCH}, c = r - ly * {NCH};")
-            w(f"            float *dl = ld + {a_expr('a', HW)} + ly * {W} + c * {V};")
+            w(f"            float *dl = ld + {'2 * (' if IL else '('}{a_expr('a', HW)} + ly * {W} + c * {V});")
             if conds:
                 # outside an open box the LDS holds zeros (no spin there)
                 w(f"            const int gxu = tx0 - {HX} + c * {V}, gyu = ty0 - {HY} + ly;")
                 w(f"            const bool ok = {' && '.join(conds)};")
-            for k in range(n):
-                val = f"(ok ? p{u}_{k} : {zero})" if conds else f"p{u}_{k}"
-                w(f"            *({VT} *)(dl + {k * NC}) = {val};")
+            vals = [f"(ok ? p{u}_{k} : {zero})" if conds else f"p{u}_{k}" for k in range(n)]
+            if IL:
+                # four cells' (x, y) pairs as two float4 stores
+                w(f"            const f32x4 q0 = {vals[0]}, q1 = {vals[1]};")
+                w("            *(f32x4 *)dl = (f32x4){q0.x, q1.x, q0.y, q1.y};")
+                w("            *(f32x4 *)(dl + 4) = (f32x4){q0.z, q1.z, q0.w, q1.w};")
+            else:
+                for k in range(n):
+                    w(f"            *({VT} *)(dl + {k * NC}) = {vals[k]};")
             if guard:
                 w("            }")
             w("        }")
@@ -1306,12 +1318,17 @@ class _Gen:
         w(f"            const int uy = r / {NCI}, c = r - uy * {NCI};")
         w(f"            const int gx = tx0 + c * {V}, gy = ty0 + uy;")
         w(f"            if (gx >= {self.L1} || gy >= {self.L2}) continue;")
-        w(f"            const float *sl = ld + {a_expr('a', HW)} + (uy + {HY}) * {W} + c * {V} + {HX};")
+        w(f"            const float *sl = ld + {'2 * (' if IL else '('}{a_expr('a', HW)} + (uy + {HY}) * {W} + c * {V} + {HX});")
         w(f"            const int64_t q = (int64_t)a * {self.PL} + (int64_t)gy * {self.L1} + gx;")
         # non-temporal: the written tile is not read again in this sweep, so its lines do not displace the halo
         # rows that the next tiles read from L2 (+2 % at XY 2048^2 x 2048, one box)
-        for k in range(n):
-            w(f"            __builtin_nontemporal_store(*(const {VT} *)(sl + {k * NC}), ({VT} *)(D + {self.NS * k} + q));")
+        if IL:
+            w("            const f32x4 r0 = *(const f32x4 *)sl, r1 = *(const f32x4 *)(sl + 4);")
+            w("            __builtin_nontemporal_store((f32x4){r0.x, r0.z, r1.x, r1.z}, (f32x4 *)(D + q));")
+            w(f"            __builtin_nontemporal_store((f32x4){{r0.y, r0.w, r1.y, r1.w}}, (f32x4 *)(D + {self.NS} + q));")
+        else:
+            for k in range(n):
+                w(f"            __builtin_nontemporal_store(*(const {VT} *)(sl + {k * NC}), ({VT} *)(D + {self.NS * k} + q));")
         w("        }")
         w("    }")
         w("}")
