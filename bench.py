@@ -445,6 +445,7 @@ def nanodisk_main(a) -> None:
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("nanodisks"),
                         "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6,
                         "launches_timed": kn},
+           "valu": valu_from_profiles("nanodisks"),
            "autotune": [[lbl, ms] for lbl, ms in Bt.tuning], "wall_s": wall}
     if not a.no_cpu_baseline and sweeps:
         ncores = min(16, reps, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1)

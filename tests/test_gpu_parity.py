@@ -159,6 +159,53 @@ def test_masked_sites_stay_zero_and_unit_norm_elsewhere():
     assert torch.allclose(norms[:, m], torch.ones_like(norms[:, m]), atol=1e-5)
 
 
+def test_fully_masked_lattice_has_no_classes():
+    """A lattice whose every site is masked has no classes (SCK/sampler.py:127-129 finds none): sweeps launch no
+    class kernel, leave the spins zero and still advance the sweep counter, and E = M = 0."""
+    s = sy.System("empty square", 1, cr.make("square"), (8, 8), (True, True), {1: 1.0},
+                  mask=np.zeros((1, 1, 8, 8), bool))
+    for kw in (dict(packed=False), dict(packed=False, spin_dtype=torch.float32)):
+        A = Sampler(s, 4, DEV, seed=1, **kw)
+        assert A.classes == [] and s.n_active == 0
+        A.set_temperature(2.0)
+        S = A.init_spins(seed=0)
+        assert not torch.any(S != 0)
+        A.sweep(S, 3)
+        E, M = A.observe(S)
+        assert not torch.any(S != 0) and int(A.state[1]) == 3
+        assert torch.all(E == 0) and torch.all(M == 0)
+
+
+def test_single_replica_batches_match_oracle():
+    """B = 1 in every storage: int8 Ising bit-exact against the sequential oracle, a bitwise word holding one
+    live replica (31 padding lanes) against the bitwise oracle, and Heisenberg decisions within the tolerance."""
+    s = sy.ising("sc", 6)
+    A = Sampler(s, 1, DEV, seed=4, packed=False)
+    A.set_temperature(3.0)
+    S = A.init_spins(seed=2)
+    S0 = S.cpu().numpy()
+    A.sweep(S, 4)
+    ref, _ = O.chain(O.OracleSystem(s), S0.astype(np.float32), A.beta.cpu().numpy(), 4, 0, 4)
+    assert np.array_equal(S.cpu().numpy().astype(np.float32), ref)
+    W = Sampler(s, 1, DEV, seed=4, bitwise=True, independent=True)
+    assert W.W == 1 and W.Bp == 32
+    W.set_temperature(3.0)
+    Sw = W.init_spins(seed=2)
+    x0 = O.unpack(Sw.cpu().numpy(), 1, s.site_mask())[:, 0]
+    W.sweep(Sw, 4)
+    xw = O.bitwise_chain(O.OracleSystem(s), x0, W.ptab.cpu().numpy(), 4, 0, 4, independent=True)
+    assert np.array_equal(O.unpack(Sw.cpu().numpy(), 1, s.site_mask())[:, 0], xw)
+    h = sy.heisenberg("bcc", 4)
+    H = Sampler(h, 1, DEV, seed=4)
+    H.set_temperature(1.2)
+    Sh = H.init_spins(seed=2)
+    h0 = Sh.cpu().numpy()
+    H.sweep(Sh, 1, classes=(0, 1))
+    x, _, mg = O.chain(O.OracleSystem(h), h0, H.beta.cpu().numpy(), 4, 0, 1, class_range=(0, 1), margins=True)
+    bad = np.abs(Sh.cpu().numpy() - x).max(axis=1) > 1e-5
+    assert not np.any(bad & ~(np.abs(mg.reshape(bad.shape)) < 1e-5))
+
+
 # ------------------------------------------------------------------------------ against the CPU oracle
 def test_ising_c1_shape_bit_exact_against_oracle():
     """Ising square 64^2, int8, B = 512 (8 temperatures x 64 replicas in [1.5, 3.5]), seed 0: 30 sweeps on the GPU,
