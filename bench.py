@@ -73,6 +73,13 @@ REF_H100 = {
 }
 REF_H100_BATCHES = (8, 16, 32, 64, 128)
 REF_H100_SCAN = {512: 572.8, 1024: 698.8, 2048: 745.2, 4096: 761.1}
+# single-replica large lattices: the chunked-row check (e1-chunk-exactness.csv:2-5, "Chunked flips/ns": eager
+# 3-sweep wall-clock rate, seed 7, T = 1, initial spins seed 8) and the reference kernel's own side of its
+# CUDA-METRO pairing (e6-cudametro-v17.csv:2,6,10, float32: Heisenberg triangular ferromagnet, T = 1, seed 6007,
+# graph replays of k = ceil(2e7 / N) sweeps)
+REF_H100_LARGE = {("heisenberg", "triangular", 2048): 3.993, ("heisenberg", "triangular", 4096): 4.43,
+                  ("xy", "square", 2048): 5.3, ("heisenberg", "sc-rows", 1152): 0.41}
+REF_H100_METRO = {1024: 11.747, 2048: 12.11, 4096: 12.585}
 
 
 def _make(workload: str):
@@ -267,8 +274,10 @@ def reference_table_main(a) -> None:
     B = 8, 16, 32, 64, 128 with the Sampler's default options (packed storage for Ising from B = 32, as the
     reference), T = the system's reference temperature, seed 1, graph replays of k = ceil(2e7 / (B N))
     sweeps; then the 2-D bitwise scan of SCK/exp_perf.py:202-231 (Ising square L = side / 8, B = 64,
-    independent draws, T = Tc, seed 5011). One JSON line per row with the published H100 rate beside it,
-    then the geometric mean of the ratios."""
+    independent draws, T = Tc, seed 5011); then the single-replica large lattices of the reference's
+    chunked-row check (SCK/chunk_check.py) and of its own kernel's side of the CUDA-METRO pairing
+    (SCK/cudametro_pair.py), each with that experiment's protocol. One JSON line per row with the published H100
+    rate beside it, then the geometric mean of the ratios."""
     import math
 
     import numpy as np
@@ -318,7 +327,46 @@ def reference_table_main(a) -> None:
                           "flips_per_ns": r, "h100_flips_per_ns": ref, "ratio": r / ref}), flush=True)
         del A, S
         torch.cuda.empty_cache()
-    print(json.dumps({"benchmark": "reference throughput table and bitwise scan", "rows": len(ratios),
+    from sck_amd import crystals as cr
+    for (kind, crystal, L), ref in REF_H100_LARGE.items():
+        # SCK/chunk_check.py:21-50: B = 1, four sweeps, then the wall-clock rate of three eager sweeps
+        if crystal == "sc-rows":
+            s = sy.System("Heisenberg sc long rows", 3, cr.make("sc"), (L, 8, 8), (True,) * 3, {1: 1.0})
+        else:
+            s = (sy.heisenberg if kind == "heisenberg" else sy.xy)(crystal, L)
+        A = Sampler(s, 1, "cuda:0", 7)
+        A.set_temperature(1.0)
+        S = A.init_spins(seed=8)
+        A.sweep(S, 4)
+        rs = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            A.sweep(S, 3)
+            torch.cuda.synchronize()
+            rs.append(s.n_active * 3 / ((time.perf_counter() - t0) * 1e9))
+        r = float(np.median(rs))
+        ratios.append(r / ref)
+        print(json.dumps({"benchmark": "reference large-lattice rows", "system": s.name, "box": list(s.box), "batch": 1,
+                          "kernel": "tiled" if A.use_tiled else A.layout, "flips_per_ns": r,
+                          "h100_flips_per_ns": ref, "ratio": r / ref}), flush=True)
+        del A, S
+        torch.cuda.empty_cache()
+    for L, ref in REF_H100_METRO.items():
+        # SCK/cudametro_pair.py:368-400: the ferromagnet, B = 1 in float32, T = 1, seed 6007
+        s = sy.heisenberg("triangular", L)
+        A = Sampler(s, 1, "cuda:0", 6007)
+        A.set_temperature(1.0)
+        S = A.init_spins(seed=6007)
+        k = max(1, int(math.ceil(2e7 / s.n_active)))
+        r = _graph_rate(A, S, k, s.n_active * k, reps)
+        ratios.append(r / ref)
+        print(json.dumps({"benchmark": "reference kernel of the CUDA-METRO pairing", "system": s.name, "box": list(s.box),
+                          "batch": 1, "kernel": "tiled" if A.use_tiled else A.layout, "flips_per_ns": r,
+                          "h100_flips_per_ns": ref, "ratio": r / ref}), flush=True)
+        del A, S
+        torch.cuda.empty_cache()
+    print(json.dumps({"benchmark": "reference throughput table, bitwise scan and large lattices", "rows": len(ratios),
                       "geomean_ratio_vs_h100": float(np.exp(np.mean(np.log(ratios)))),
                       "min_ratio": float(min(ratios)), "max_ratio": float(max(ratios))}), flush=True)
 
