@@ -36,7 +36,9 @@ enum sck_mode {
     SCK_MODE_F32 = 1,           /* float32 components (Ising stored as float, XY, Heisenberg) */
     SCK_MODE_PACKED = 2,        /* Ising, 32 replicas per int32 word, per-replica Philox draws */
     SCK_MODE_BITWISE = 3,       /* bitwise multi-spin Ising, one uniform per (word, site) */
-    SCK_MODE_BITWISE_INDEP = 4  /* bitwise multi-spin Ising, one 32-bit U per replica */
+    SCK_MODE_BITWISE_INDEP = 4, /* bitwise multi-spin Ising, one 32-bit U per replica */
+    SCK_MODE_F64 = 5            /* float64 components, updated in float32 arithmetic: the reference's storage
+                                   dtype cast (SCK/generator.py:214 loads as float32, :310-319 stores the cast) */
 };
 
 /* Arguments every generated kernel receives, by value (one struct parameter).

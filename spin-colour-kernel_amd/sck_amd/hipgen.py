@@ -51,7 +51,7 @@ __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT
            "MODE_BITWISE", "MODE_BITWISE_INDEP", "BLOCK", "BatchModule", "generate_batch", "batch_stride",
            "BATCH_HDR", "CHAIN_BLOCKS", "COORD_BITS"]
 
-MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP = range(5)
+MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP, MODE_F64 = range(6)
 BLOCK = 256
 XCDS = 8                      # XCDs of an MI355X; consecutive workgroups are dispatched to them round-robin
 RESIDENT_GROUP = 4            # resident Ising kernel: sites a thread reads before it draws any of them
@@ -390,7 +390,7 @@ class _Gen:
             self.RESIDENT_THREADS = min(1024, max(256, 64 * -(-(self.ROWS * self.BX) // 64)))
         if self.rm_SP not in (1, BLOCK // 64):
             raise ValueError(f"replica-major sites per workgroup must be 1 or {BLOCK // 64}")
-        esz = 1 if mode == MODE_INT8 else 4
+        esz = {MODE_INT8: 1, MODE_F64: 8}.get(mode, 4)
         self.rm_block = self.B
         if self.replica_major:
             if rm_block is not None and int(rm_block) not in (0, self.B):
@@ -406,7 +406,7 @@ class _Gen:
                     if fits:
                         self.rm_block = max(fits)
         # one replica's spins addressed by a 32-bit byte offset from its base pointer
-        self.u32 = mode in (MODE_INT8, MODE_F32) and self.n * self.NS * esz < 2 ** 31
+        self.u32 = mode in (MODE_INT8, MODE_F32, MODE_F64) and self.n * self.NS * esz < 2 ** 31
         self.rt = False               # box dimensions are compile-time constants (runtime: batch kernels)
         self.lines: list[str] = []
 
@@ -428,7 +428,7 @@ class _Gen:
             return f"rm_load({ptr}, (int64_t)({site}), (uint32_t)rbase, {comp})"
         if self.u32:
             # the replica's base pointer (wave-uniform wherever the replica is) plus a 32-bit byte offset
-            esz = 1 if self.mode == MODE_INT8 else 4
+            esz = {MODE_INT8: 1, MODE_F64: 8}.get(self.mode, 4)
             return (f"(*(decltype({ptr}))((const char *)({ptr} + rbase) + "
                     f"(uint32_t)({self._mul(comp, self.NS)} + {site}) * {esz}u))")
         return f"{ptr}[rbase + (int64_t)({self._mul(comp, self.NS)} + {site})]"
@@ -444,7 +444,7 @@ class _Gen:
         (scalar registers), the component's offset as the scalar offset and the lane's replica as the only
         vector offset, so an access costs no vector address arithmetic."""
         w = self.w
-        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        T = self.elt()
         esz = 1 if self.mode == MODE_INT8 else 4
         w(f"DEV __amdgpu_buffer_rsrc_t rm_site(const {T} *base, int64_t site) {{")
         w(f"    return __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)base + site * {self.n * self.B * esz}), "
@@ -473,6 +473,10 @@ class _Gen:
 
     def w(self, s: str = "") -> None:
         self.lines.append(s)
+
+    def elt(self) -> str:
+        """The C type of one stored spin component (unpacked storage)."""
+        return {MODE_INT8: "int8_t", MODE_F64: "double"}.get(self.mode, "float")
 
     def _mul(self, k: int, X) -> str:
         """k * X for a compile-time X (folded) or a runtime one (an expression of the batch dimensions)."""
@@ -542,7 +546,7 @@ class _Gen:
         """__device__ update of one site of sublattice a for one replica (unpacked storage).
         Returns 1 when the move is accepted. `src` is read, `dst` written (in place: src == dst)."""
         s, n = self.s, self.n
-        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        T = self.elt()
         w = self.w
         groups, extras = _groups(s, a)
         if n == 1:
@@ -657,6 +661,12 @@ class _Gen:
         w("    if (acc) {")
         for k in range(n):
             w(f"        {store(k, f't{k}')}")
+        if self.mode == MODE_F64:
+            # the reference stores where(acc, t, s) cast to the storage type, s being the spin as loaded in
+            # float32 (SCK/generator.py:310-319): a rejected site keeps its value rounded to float32
+            w("    } else {")
+            for k in range(n):
+                w(f"        {store(k, f's{k}')}")
         w("    }")
 
     def emit_local_field(self, a: int, ind: str = "    ", nb=None) -> None:
@@ -824,7 +834,7 @@ class _Gen:
     # ------------------------------------------------------------------------------- plain kernels
     def emit_plain(self) -> None:
         w = self.w
-        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        T = self.elt()
         if self.replica_major:
             self.emit_rm_access()
         for a in range(self.b):
@@ -1375,7 +1385,7 @@ class _Gen:
         the neighbours of up to RESIDENT_GROUP of them before it decides or draws any (17 % faster at that shape,
         2-6 % on 192^2 triangular and 32^3 sc lattices at B = 128)."""
         w = self.w
-        T = "int8_t" if self.mode == MODE_INT8 else "float"
+        T = self.elt()
         RT = self.RESIDENT_THREADS
         chunk, lds = plan
         NSn = self.n * self.NS
@@ -2004,7 +2014,7 @@ class _Gen:
             w("    const double sv = ((wv >> (rep & 31)) & 1u) ? 1.0 : -1.0;")
             w("    return A.mask[q] ? sv : 0.0;")
         else:
-            T = "int8_t" if self.mode == MODE_INT8 else "float"
+            T = self.elt()
             if self.replica_major:
                 w(f"    return (double)((const {T} *)A.spins)[((int64_t)q * {self.n} + comp) * {self.B} + rep];")
             else:
@@ -2191,7 +2201,7 @@ class _Gen:
         if self.want_tiled and tplan is None:
             raise ValueError("the tiled sweep needs float32 vector spins on a 2-D lattice in the reference layout, "
                              "larger than a tile and its halo")
-        if self.mode in (MODE_INT8, MODE_F32):
+        if self.mode in (MODE_INT8, MODE_F32, MODE_F64):
             self.emit_plain()
             if rplan is not None:
                 self.emit_resident(rplan)
@@ -2220,7 +2230,7 @@ class _Gen:
                   [f"sck_sync_{a}" for a in range(self.b)] + ["sck_obs", "sck_obs_final", "sck_bump"] + \
                   (["sck_resident"] if rplan is not None else []) + \
                   (["sck_tiled", "sck_track_fold"] if tplan is not None else []) + \
-                  (["sck_sample"] if (self.n > 1 and self.mode == MODE_F32) else [])
+                  (["sck_sample"] if (self.n > 1 and self.mode in (MODE_F32, MODE_F64)) else [])
         ki = {k: i for i, k in enumerate(kernels)}
         R = self.W if self.mode in (MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP) else self.B
         gy, gz = (R, 1) if R <= 65535 else (65535, -(-R // 65535))
@@ -2316,7 +2326,7 @@ class _BatchGen(_Gen):
         self.rt = True
         self.L1, self.L2, self.L3, self.PL, self.NS = "D.L1", "D.L2", "D.L3", "D.PL", "D.NS"
         self.wide = bool(wide)
-        self.T = "int8_t" if mode == MODE_INT8 else "float"
+        self.T = self.elt()
         self.stride = batch_stride(len(self.classes))
 
     def emit_batch_helpers(self) -> None:

@@ -2,8 +2,8 @@
 
 It replaces `generator.compile_system` (SCK/generator.py:322-337) and the per-class Triton launch loop of
 `Sampler.sweep` (SCK/sampler.py:226-240) with libsck's C-ABI (include/sck.h). It reads only what the reference's
-own Sampler holds (SCK/sampler.py:40-131): `sys, packed, bitwise, independent, B, W, Bp, mask, k1, beta, state`
-and, for bitwise storage, `ptab`. Its System may carry only the reference's fields (SCK/system.py:19-34).
+own Sampler holds (SCK/sampler.py:40-131): `sys, packed, bitwise, independent, spin_dtype, B, W, Bp, mask, k1, beta,
+state` and, for bitwise storage, `ptab`. Its System may carry only the reference's fields (SCK/system.py:19-34).
 
 What the reference keeps no counterpart of, the binding owns:
   * the Ising acceptance table. The reference kernel evaluates exp in place (SCK/generator.py:303-309); the
@@ -52,10 +52,11 @@ class MI355XKernel:
             self.mode = hipgen.MODE_BITWISE_INDEP if sampler.independent else hipgen.MODE_BITWISE
         elif sampler.packed:
             self.mode = hipgen.MODE_PACKED
-        elif s.n == 1 and getattr(sampler, "spin_dtype", torch.int8) == torch.int8:
-            self.mode = hipgen.MODE_INT8
         else:
-            self.mode = hipgen.MODE_F32
+            # the reference's storage dtype (SCK/sampler.py:66): int8 Ising, float32, or float64 (updated in
+            # float32 arithmetic, as the reference loads any storage type)
+            dt = getattr(sampler, "spin_dtype", torch.int8 if s.n == 1 else torch.float32)
+            self.mode = {torch.int8: hipgen.MODE_INT8, torch.float64: hipgen.MODE_F64}.get(dt, hipgen.MODE_F32)
         self.B, self.W = int(sampler.B), int(sampler.W)
         self.packed = self.mode in (hipgen.MODE_PACKED, hipgen.MODE_BITWISE, hipgen.MODE_BITWISE_INDEP)
         # the reference layout [B, n, b, L3, L2, L1]: the reference's tensors are used as they are

@@ -139,8 +139,8 @@ class Sampler:
         if self.packed and system.n != 1:
             raise ValueError("bit packing is for Ising (n = 1) only")
         self.spin_dtype = spin_dtype or (torch.int8 if system.n == 1 else torch.float32)
-        if not self.packed and self.spin_dtype not in (torch.int8, torch.float32):
-            raise ValueError("spins are stored as int8 (Ising) or float32")
+        if not self.packed and self.spin_dtype not in (torch.int8, torch.float32, torch.float64):
+            raise ValueError("spins are stored as int8 (Ising), float32 or float64")
         if self.spin_dtype == torch.int8 and system.n != 1:
             raise ValueError("int8 storage holds Ising spins only")
         self.B = int(batch)
@@ -172,7 +172,8 @@ class Sampler:
         elif self.packed:
             self.mode = G.MODE_PACKED
         else:
-            self.mode = G.MODE_INT8 if self.spin_dtype == torch.int8 else G.MODE_F32
+            # float64 storage is updated in float32 arithmetic, as the reference loads any storage type
+            self.mode = {torch.int8: G.MODE_INT8, torch.float64: G.MODE_F64}.get(self.spin_dtype, G.MODE_F32)
         self.learned_meta = learned.meta() if learned is not None else None
         if layout not in ("auto", "reference", "replica"):
             raise ValueError("layout must be 'auto', 'reference' or 'replica'")

@@ -159,6 +159,55 @@ def test_masked_sites_stay_zero_and_unit_norm_elsewhere():
     assert torch.allclose(norms[:, m], torch.ones_like(norms[:, m]), atol=1e-5)
 
 
+@pytest.mark.parametrize("make", [
+    lambda: sy.xy("triangular", 12),
+    lambda: sy.heisenberg("bcc", 4),
+    lambda: sy.nanodot(Rd=2.0, L=2),
+    lambda: sy.ising("sc", 5),
+])
+def test_float64_storage_equals_float32(make):
+    """spin_dtype=torch.float64, as the reference allows (its kernel loads any storage type as float32 and stores
+    where(acc, t, s) cast back, SCK/generator.py:214, 310-319): sweeps, the sequential chain, the synchronous
+    control, graph replays and the observables equal those of float32 storage exactly, and a sweep leaves every
+    stored value a float32 value even where the move was rejected."""
+    s = make()
+    B = 6
+    F = Sampler(s, B, DEV, seed=3, packed=False, spin_dtype=torch.float32)
+    D = Sampler(s, B, DEV, seed=3, packed=False, spin_dtype=torch.float64)
+    assert D.mode != F.mode and D.layout == "reference" and not D.use_tiled
+    T = torch.linspace(0.6, 2.4, B)
+    for X in (F, D):
+        X.set_temperature(T)
+    Sf = F.init_spins(seed=5)
+    Sd = D.init_spins(seed=5)
+    assert Sd.dtype == torch.float64 and torch.equal(Sd, Sf.double())
+    F.sweep(Sf, 3)
+    D.sweep(Sd, 3)
+    assert torch.equal(Sd, Sf.double())
+    Ef, Mf = F.observe(Sf)
+    Ed, Md = D.observe(Sd)
+    assert torch.equal(Ed, Ef) and torch.equal(Md, Mf)
+    rf, rd = F.graph(Sf, 2), D.graph(Sd, 2)
+    rf()
+    rd()
+    torch.cuda.synchronize()
+    assert torch.equal(Sd, Sf.double())
+    Rf, Rd = Sf.clone(), Sd.clone()
+    F.reference_sweep(Rf, 1)
+    D.reference_sweep(Rd, 1)
+    assert torch.equal(Rd, Rf.double())
+    F.sync_sweep(Sf, 1)
+    D.sync_sweep(Sd, 1)
+    assert torch.equal(Sd, Sf.double())
+    if s.n > 1:
+        # float64 values that float32 cannot hold: after one sweep every live site holds a float32 value
+        X = torch.randn((B, s.n) + s.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+        X = (X / X.norm(dim=1, keepdim=True)).to(DEV) * torch.as_tensor(s.site_mask(), device=DEV)
+        assert not torch.equal(X, X.float().double())
+        D.sweep(X, 1)
+        assert torch.equal(X, X.float().double())
+
+
 def test_fully_masked_lattice_has_no_classes():
     """A lattice whose every site is masked has no classes (SCK/sampler.py:127-129 finds none): sweeps launch no
     class kernel, leave the spins zero and still advance the sweep counter, and E = M = 0."""

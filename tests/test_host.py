@@ -261,6 +261,30 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
+def test_storage_modes_match_the_c_header():
+    """The generator's storage modes carry the values of enum sck_mode in include/sck.h."""
+    import re
+    from sck_amd import hipgen as G
+    hdr = (Path(__file__).resolve().parents[1] / "include" / "sck.h").read_text()
+    enum = {m.group(1): int(m.group(2)) for m in re.finditer(r"SCK_MODE_(\w+) = (\d+)", hdr)}
+    assert enum == {"INT8": G.MODE_INT8, "F32": G.MODE_F32, "PACKED": G.MODE_PACKED, "BITWISE": G.MODE_BITWISE,
+                    "BITWISE_INDEP": G.MODE_BITWISE_INDEP, "F64": G.MODE_F64}
+
+
+def test_float64_storage_runs_the_reference_layout():
+    """float64 storage (updated in float32 arithmetic) generates the plain kernels with double loads and stores,
+    stores the float32 spin back on rejection, and takes neither the replica-major, resident nor tiled kernels."""
+    from sck_amd import hipgen as G
+    from sck_amd.sampler import choose_layout
+    s = sy.xy("triangular", 2048)
+    assert choose_layout(s, G.MODE_F64, 2048) == ("reference", False)
+    m = G.generate(sy.heisenberg("bcc", 4), G.MODE_F64, 4)
+    assert m.resident is None and m.tiled is None and "sck_sample" in m.kernels
+    upd = m.src[m.src.index("DEV int upd_0"):]
+    upd = upd[:upd.index("\n}\n")]
+    assert "const double *__restrict__ src" in upd and "} else {" in upd
+
+
 @pytest.mark.parametrize("make, msg", [
     (lambda: sy.heisenberg("sc", 4), "Ising"),
     (lambda: sy.System("J1-J2", 1, cr.make("sc", (1, 2)), (4, 4, 4), (True,) * 3, {1: 1.0, 2: 0.5}), "uniform"),
