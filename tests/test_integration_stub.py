@@ -93,7 +93,8 @@ class RefSampler:
     """The attributes the reference Sampler holds after __init__ (SCK/sampler.py:40-131) and its
     set_temperature (SCK/sampler.py:134-146), plus the one line that calls the binding."""
 
-    def __init__(self, system, batch, device="cuda", seed=0, packed=None, bitwise=False, independent=False):
+    def __init__(self, system, batch, device="cuda", seed=0, packed=None, bitwise=False, independent=False,
+                 spin_dtype=None):
         from sck_amd.mi355x import MI355XKernel
         from sck_amd.sampler import p_fixed, p_table
         self.sys = system
@@ -101,7 +102,7 @@ class RefSampler:
         if self.bitwise:
             packed = True
         self.packed = (system.n == 1 and batch >= WORD) if packed is None else bool(packed)
-        self.spin_dtype = torch.int8 if system.n == 1 else torch.float32
+        self.spin_dtype = spin_dtype or (torch.int8 if system.n == 1 else torch.float32)
         self.B = int(batch)
         self.device = torch.device(device)
         self.W = -(-self.B // WORD) if self.packed else self.B
@@ -206,6 +207,26 @@ def test_binding_reproduces_reference_trajectories(name):
         E = A._mi355x.energy(A, S).cpu().numpy()
         M = A._mi355x.magnetization(A, S).cpu().numpy()
         assert np.allclose(E, z["E"], rtol=1e-9, atol=1e-4) and np.allclose(M, z["M"], rtol=1e-9, atol=1e-4)
+
+
+@pytest.mark.gpu
+def test_binding_float64_storage_follows_the_reference_trajectory():
+    """A reference Sampler holding float64 spins (spin_dtype=torch.float64; its kernel loads them as float32 and
+    stores the cast, SCK/generator.py:214, 310-319): the binding runs float64 storage and follows the
+    reference's xy_square4 trajectory."""
+    from sck_amd import hipgen
+    rc, r = _recipe_system("xy_square4")
+    z = np.load(GOLDEN / "xy_square4.npz")
+    A = RefSampler(r, rc["B"], "cuda", seed=rc["seed"], spin_dtype=torch.float64)
+    assert A._mi355x.mode == hipgen.MODE_F64
+    A.set_temperature(torch.as_tensor(rc["T"], dtype=torch.float32))
+    S = torch.as_tensor(z["S0"]).to("cuda", torch.float64).contiguous()
+    for k in range(rc["sweeps"]):
+        A.sweep(S, 1)
+        assert S.dtype == torch.float64
+        assert np.allclose(S.cpu().numpy(), z["S"][k], rtol=0, atol=1e-5), f"sweep {k}"
+    E = A._mi355x.energy(A, S).cpu().numpy()
+    assert np.allclose(E, z["E"], rtol=1e-9, atol=1e-4)
 
 
 @pytest.mark.gpu
