@@ -273,6 +273,25 @@ def test_ising_c1_shape_bit_exact_against_oracle():
         assert np.array_equal(S.cpu().numpy()[reps].astype(np.float32), ref), f"resident={resident}"
 
 
+def test_ising_c1_config_1000_sweeps_bit_exact():
+    """The whole C1 configuration: Ising square 64^2, int8, B = 512 (8 temperatures x 64 in [1.5, 3.5]), seed 0,
+    1000 sweeps in one graph replay on the resident kernel; replicas 0 and 511 (the coldest and the hottest)
+    equal the sequential oracle's 1000-sweep chains on the same counters, spin for spin."""
+    s = sy.ising("square", 64)
+    T = np.repeat(np.linspace(1.5, 3.5, 8).astype(np.float32), 64)
+    A = Sampler(s, 512, DEV, seed=0, packed=False)
+    A.set_temperature(torch.as_tensor(T))
+    S = A.init_spins(seed=0)
+    S0 = S.cpu().numpy()
+    rp = A.graph(S, 1000)
+    rp()
+    torch.cuda.synchronize()
+    assert int(A.state[1]) == 1000
+    reps = [0, 511]
+    ref, _ = O.chain(O.OracleSystem(s), S0[reps].astype(np.float32), A.beta.cpu().numpy(), 0, 0, 1000, reps=reps)
+    assert np.array_equal(S.cpu().numpy()[reps].astype(np.float32), ref)
+
+
 @pytest.mark.parametrize("workload", ["heisenberg", "xy"])
 def test_full_size_vector_configs_match_oracle(workload):
     """The replica-major kernels at their real size: Heisenberg bcc 32^3, J1 = 1, J2 = -0.2, Du = 0.1, h = 0.05,
