@@ -208,6 +208,30 @@ def test_float64_storage_equals_float32(make):
         assert torch.equal(X, X.float().double())
 
 
+@pytest.mark.parametrize("make, B, kw", [
+    (lambda: sy.ising("square", 4), 3, {}), (lambda: sy.xy("square", 4), 3, {}),
+    (lambda: sy.heisenberg("bcc", 2), 3, {}), (lambda: sy.heisenberg("kagome", 3, J=-1.0), 3, {}),
+    (lambda: sy.nanodot(Rd=2.0, L=2), 3, {}),
+    (lambda: sy.xy("triangular", 128), 8, dict(tiled=True)),
+    (lambda: sy.heisenberg("bcc", 6), 64, dict(layout="replica", resident=False)),
+    (lambda: sy.ising("sc", 32), 32, dict(bitwise=True, independent=True)),
+])
+def test_zero_temperature_never_raises_energy(make, B, kw):
+    """At beta -> infinity only dE <= 0 is accepted, so the energy cannot rise (the reference's
+    tests/test_core.py:58-68 invariant, here on every kernel family: resident and class launches, the tiled
+    sweep, the replica-major layout and the bitwise words)."""
+    s = make()
+    A = Sampler(s, B, DEV, seed=5, **kw)
+    A.set_temperature(1e-6)
+    S = A.init_spins(seed=1)
+    E = A.energy(S)
+    for _ in range(3):
+        A.sweep(S)
+        E2 = A.energy(S)
+        assert torch.all(E2 <= E + 1e-4)
+        E = E2
+
+
 def test_fully_masked_lattice_has_no_classes():
     """A lattice whose every site is masked has no classes (SCK/sampler.py:127-129 finds none): sweeps launch no
     class kernel, leave the spins zero and still advance the sweep counter, and E = M = 0."""
