@@ -232,6 +232,32 @@ def test_zero_temperature_never_raises_energy(make, B, kw):
         E = E2
 
 
+@pytest.mark.parametrize("make, base", [
+    (lambda: sy.ising("square", 16), dict(bitwise=True)),
+    (lambda: sy.ising("sc", 6), dict(bitwise=True, independent=True)),
+    (lambda: sy.xy("square", 24), dict()),
+])
+def test_launch_options_change_no_result(make, base):
+    """The reference's launch options (words per program, bitwise tile and warps, row compaction, max_block:
+    SCK/sampler.py:40-131) choose launch geometry only; its tests check the stream is independent of them
+    (tests/test_bitwise.py:84-97, 229-265). Every option gives the default's spins here too."""
+    s = make()
+    B = 96
+    T = torch.linspace(1.2, 3.0, B // 32 if base.get("bitwise") else B)
+    if base.get("bitwise"):
+        T = T.repeat_interleave(32)
+    runs = []
+    for kw in ({}, dict(words_per_program=1), dict(words_per_program=4), dict(words_per_program=8),
+               dict(bitwise_tile=64, bitwise_warps=2), dict(compact=False), dict(max_block=16)):
+        A = Sampler(s, B, DEV, seed=11, **base, **kw)
+        A.set_temperature(T)
+        S = A.init_spins(seed=4)
+        A.sweep(S, 3)
+        runs.append(S.cpu())
+    for i, r in enumerate(runs[1:], 1):
+        assert torch.equal(r, runs[0]), i
+
+
 def test_fully_masked_lattice_has_no_classes():
     """A lattice whose every site is masked has no classes (SCK/sampler.py:127-129 finds none): sweeps launch no
     class kernel, leave the spins zero and still advance the sweep counter, and E = M = 0."""
