@@ -258,6 +258,44 @@ def test_launch_options_change_no_result(make, base):
         assert torch.equal(r, runs[0]), i
 
 
+def test_independent_bitwise_replicas_do_not_coalesce():
+    """Ising square 8x8 at Tc, Barker, one word of 32 replicas, shared against independent draws (the reference's
+    tests/test_bitwise.py:290-318). (a) From 32 identical configurations the shared draw keeps one configuration
+    and the independent draws give at least 30 after 20 sweeps. (b) From random starts the shared draw collapses
+    the word to a few configurations within 400 sweeps, while independent replicas share a configuration only
+    where it is fully ordered."""
+    s = sy.ising("square", 8)
+    s.acceptance = "barker"
+    tc = 2.0 / np.log(1.0 + np.sqrt(2.0))
+
+    def sampler(independent):
+        A = Sampler(s, 32, DEV, seed=5, bitwise=True, independent=independent)
+        A.set_temperature(tc)
+        return A
+
+    def distinct(A, S):
+        return len(np.unique(A.unpack(S)[:, 0].reshape(32, -1).cpu().numpy(), axis=0))
+
+    res = {}
+    for ind in (False, True):
+        A = sampler(ind)
+        x = A.unpack(A.init_spins(seed=2))
+        S = A.pack(x[:1].expand_as(x).float())
+        assert distinct(A, S) == 1
+        A.sweep(S, 20)
+        res[ind, "same"] = distinct(A, S)
+        A = sampler(ind)
+        S = A.init_spins(seed=2)
+        assert distinct(A, S) == 32
+        A.sweep(S, 400)
+        res[ind, "random"] = distinct(A, S)
+        x = A.unpack(S)[:, 0].reshape(32, -1).cpu().numpy()
+        rows, counts = np.unique(x, axis=0, return_counts=True)
+        res[ind, "repeated_disordered"] = int(sum(1 for r, c in zip(rows, counts) if c > 1 and abs(r.mean()) < 1))
+    assert res[False, "same"] == 1 and res[True, "same"] >= 30, res
+    assert res[False, "random"] <= 4 and res[True, "repeated_disordered"] == 0 and res[True, "random"] >= 28, res
+
+
 def test_fully_masked_lattice_has_no_classes():
     """A lattice whose every site is masked has no classes (SCK/sampler.py:127-129 finds none): sweeps launch no
     class kernel, leave the spins zero and still advance the sweep counter, and E = M = 0."""
