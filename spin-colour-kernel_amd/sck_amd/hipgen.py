@@ -10,11 +10,23 @@ Kernels of a module (all take one `sck_args` by value):
                   sites of one row after another (the colouring's x-period P gives each row's sites as
                   x0 + P j), one block row per replica, so loads along x stay coalesced in the reference
                   layout S[B, n, b, L3, L2, L1].
+                  With the replica-major layout S[b, L3, L2, L1, n, B] the lanes cover replicas of one site
+                  instead, so every neighbour load is a contiguous run over replicas.
+  sck_tiled       whole sweeps of a 2-D vector lattice in one launch: a workgroup stages a tile and a halo of
+                  one bond length per class in LDS, updates the classes in sweep order on a halo that shrinks
+                  by one bond per class, and writes the tile's interior (draws depend only on seed, sweep,
+                  replica and site, so halo copies repeat their neighbours' updates bit for bit).
+  sck_resident    k whole sweeps of one replica per workgroup with the lattice in LDS (small lattices).
   sck_seq         the sequential chain: one thread per replica (per word for packed/bitwise storage) visiting
                   the classes in sweep order and their sites in raster order (SCK/sampler.py:281-293).
   sck_sync_<a>    synchronous control of sublattice a (SCK/sampler.py:271-279).
   sck_obs, sck_obs_final   float64 energy and magnetization: per-block partial sums, then a fixed-order sum.
+  sck_track_fold  adds one tiled sweep's per-(replica, tile) changes of E and M to the running totals.
+  sck_sample      records (spin, local field, beta, constants) at chosen (replica, site) pairs for training.
   sck_bump        state[1] += sweep_offset.
+
+A batch module (generate_batch) holds the same kernels over many uneven systems: sck_bclass_<k>, sck_bseq,
+sck_bobs, and sck_bchain_<t> for persistent launches of several sweeps.
 
 The per-site update follows SCK/generator.py:202-318 operation by operation in fp32 with contraction off, so
 the sequential kernel, the class kernels and the CPU oracle agree:
@@ -48,7 +60,7 @@ import numpy as np
 from .system import System, as_system
 
 __all__ = ["Launch", "Module", "generate", "x_period", "ising_layout", "MODE_INT8", "MODE_F32", "MODE_PACKED",
-           "MODE_BITWISE", "MODE_BITWISE_INDEP", "BLOCK", "BatchModule", "generate_batch", "batch_stride",
+           "MODE_BITWISE", "MODE_BITWISE_INDEP", "MODE_F64", "BLOCK", "BatchModule", "generate_batch", "batch_stride",
            "BATCH_HDR", "CHAIN_BLOCKS", "COORD_BITS"]
 
 MODE_INT8, MODE_F32, MODE_PACKED, MODE_BITWISE, MODE_BITWISE_INDEP, MODE_F64 = range(6)
