@@ -7,7 +7,6 @@ Every function cites the reference lines it restates. Spins cross this boundary 
 from __future__ import annotations
 
 import ctypes as C
-import os
 import subprocess
 from pathlib import Path
 

@@ -23,8 +23,6 @@ import sys
 import time
 from pathlib import Path
 
-import numpy as np
-
 from . import crystals as cr
 from . import simulate as SIM
 from .hamiltonian import MODELS, build_system

@@ -14,7 +14,6 @@ import numpy as np
 import pytest
 import torch
 
-from sck_amd import crystals as cr
 from sck_amd import system as sy
 
 pytestmark = pytest.mark.gpu
