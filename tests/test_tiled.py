@@ -39,6 +39,14 @@ def _dmi_square(L1, L2, D=0.3):
                      h=(0.0, 0.1, 0.2), mask=mask)
 
 
+def _xy_triangular_open(L1, L2):
+    """XY triangular lattice, open along x, periodic along y, with a field and a disc-shaped hole in the mask."""
+    yy, xx = np.mgrid[0:L2, 0:L1]
+    mask = ((xx - L1 / 2) ** 2 + (yy - L2 / 2) ** 2 > 30)[None, None]
+    return sy.System("XY triangular open-x masked", 2, cr.make("triangular"), (L1, L2), (False, True), {1: 1.0},
+                     h=(0.1, 0.2), mask=mask)
+
+
 SYSTEMS = {
     "xy_triangular_98": (lambda: sy.xy("triangular", 98), {}, None),
     "xy_triangular_96_3col": (lambda: sy.xy("triangular", 96), {}, None),
@@ -49,6 +57,9 @@ SYSTEMS = {
     "heis_dmi_open_masked": (lambda: _dmi_square(90, 80), {}, None),
     "xy_square_cone": (lambda: sy.xy("square", 80), {"proposal": "cone"}, None),
     "heis_triangular_barker": (lambda: sy.heisenberg("triangular", 96, J=-1.0), {"barker": True}, (32, 48)),
+    # four colours, float4 rows with a cell's two components side by side in LDS, partial tiles
+    "xy_triangular_100": (lambda: sy.xy("triangular", 100), {}, None),
+    "xy_triangular_open_masked_cone": (lambda: _xy_triangular_open(100, 64), {"proposal": "cone"}, None),
 }
 
 
