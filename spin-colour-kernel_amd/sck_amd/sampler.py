@@ -597,40 +597,46 @@ class Sampler:
         return self.observe(S)[1]
 
     def local_field(self, S: torch.Tensor) -> torch.Tensor:
-        """Bond part of the local field of every site by rolls, float64, as a reference (SCK/sampler.py:296-324)."""
+        """Bond part of the local field of every site, float64 (the quantity of SCK/sampler.py:296-324), computed
+        on the host side of the kernels as a check: each bond with a non-zero matrix gathers its neighbour spins
+        through a flat cell index (zero weight across an open face) and adds its matrix times them."""
         s = self.sys
-        S64 = self.unpack(S).double()
-        v = torch.zeros_like(S64)
-        dims = {0: 5, 1: 4, 2: 3}
+        x = self.unpack(S).double()
+        shape = x.shape                                  # (B, n, b, L3, L2, L1)
+        flat = x.reshape(shape[0], s.n, s.b, -1)
+        v = torch.zeros_like(flat)
         for i, bd in enumerate(s.crystal.bonds):
-            M = torch.as_tensor(s.bond_matrix(i), dtype=torch.float64, device=S.device)
-            if not M.any():
+            M = s.bond_matrix(i)
+            if not np.any(M):
                 continue
-            src = S64[:, :, bd.alpha2]
-            valid = torch.ones(s.shape[1:], dtype=torch.float64, device=S.device)
-            for k in range(s.crystal.D):
-                d = bd.delta[k]
-                if d == 0:
-                    continue
-                src = torch.roll(src, shifts=-d, dims=dims[k] - 1)
-                if not s.periodic[k]:
-                    idx = torch.arange(s.box[k], device=S.device)
-                    ok = ((idx + d) >= 0) & ((idx + d) < s.box[k])
-                    shp = [1, 1, 1]
-                    shp[2 - k] = s.box[k]
-                    valid = valid * ok.to(torch.float64).reshape(shp)
-            src = src * valid
-            for k in range(s.n):
-                for l in range(s.n):
-                    if M[k, l] != 0:
-                        v[:, k, bd.alpha] += M[k, l] * src[:, l]
-        return v
+            idx, weight = self._bond_gather(bd)
+            nb = flat[:, :, bd.alpha2].index_select(2, torch.as_tensor(idx, device=x.device))
+            nb = nb * torch.as_tensor(weight, device=x.device)
+            v[:, :, bd.alpha] += torch.as_tensor(M, dtype=torch.float64, device=x.device) @ nb
+        return v.reshape(shape)
+
+    def _bond_gather(self, bd) -> tuple[np.ndarray, np.ndarray]:
+        """Flat cell index (z, y, x raster) of every cell's neighbour along bond bd, and its weight: 1 inside the
+        box or across a periodic face, 0 across an open face (the index is then clamped into the box)."""
+        s = self.sys
+        dims = (s.L1, s.L2, s.L3)                        # lattice axis k -> extent
+        coords = np.indices((s.L3, s.L2, s.L1)).reshape(3, -1)[::-1].astype(np.int64)   # rows x, y, z
+        weight = np.ones(coords.shape[1])
+        for k in range(s.crystal.D):
+            c = coords[k] + int(bd.delta[k])
+            if s.periodic[k]:
+                c %= dims[k]
+            else:
+                weight *= (c >= 0) & (c < dims[k])
+                np.clip(c, 0, dims[k] - 1, out=c)
+            coords[k] = c
+        return (coords[2] * s.L2 + coords[1]) * s.L1 + coords[0], weight
 
     def _energy_torch(self, x: torch.Tensor) -> torch.Tensor:
+        """Energy of unpacked Ising spins: minus half of s . v over the bonds, minus h . s."""
         s = self.sys
-        S64 = x.double()
-        E = -0.5 * (S64 * self.local_field(x)).sum(dim=(1, 2, 3, 4, 5))
+        x64 = x.double().reshape(x.shape[0], -1)
+        E = -0.5 * (x64 * self.local_field(x).reshape(x.shape[0], -1)).sum(dim=1)
         if s.h is not None:
-            E = E - (S64 * torch.as_tensor(s.h, dtype=torch.float64, device=x.device).reshape(1, -1, 1, 1, 1, 1)).sum(
-                dim=(1, 2, 3, 4, 5))
+            E = E - float(s.h[0]) * x64.sum(dim=1)
         return E

@@ -405,3 +405,50 @@ def test_reference_table_rows_cover_the_published_systems():
     assert sorted(names) == sorted(bench.REF_H100)
     assert all(len(v) == len(bench.REF_H100_BATCHES) for v in bench.REF_H100.values())
     assert sorted(bench.REF_H100_SCAN) == [512, 1024, 2048, 4096]
+
+
+def _field_by_sites(s, S):
+    """The bond part of the local field, site by site: for every cell and bond, the neighbour's coordinates,
+    wrapped on periodic axes and dropped beyond open faces, times the bond matrix."""
+    x = np.asarray(S, np.float64)
+    v = np.zeros_like(x)
+    L = (s.L1, s.L2, s.L3)
+    for i, bd in enumerate(s.crystal.bonds):
+        M = s.bond_matrix(i)
+        for z in range(s.L3):
+            for y in range(s.L2):
+                for xx in range(s.L1):
+                    q = [xx, y, z]
+                    inside = True
+                    for k in range(s.crystal.D):
+                        q[k] += bd.delta[k]
+                        if s.periodic[k]:
+                            q[k] %= L[k]
+                        elif not 0 <= q[k] < L[k]:
+                            inside = False
+                    if inside:
+                        v[:, :, bd.alpha, z, y, xx] += x[:, :, bd.alpha2, q[2], q[1], q[0]] @ M.T
+    return v
+
+
+@pytest.mark.parametrize("make", [
+    lambda: sy.heisenberg("bcc", 3),
+    lambda: sy.System("Heis fcc open", 3, cr.make("fcc"), (3, 2, 4), (False, True, False), {1: 1.0, 2: -0.3}),
+    lambda: sy.xy("triangular", 5),
+    lambda: sy.ising("square", 5),
+], ids=["bcc", "fcc_open", "triangular", "square"])
+def test_local_field_equals_the_site_by_site_sum(make):
+    """Sampler.local_field (the float64 check quantity of SCK/sampler.py:296-324, here an index gather) against
+    the same sum written site by site; it runs on the host, so it takes a stand-in holding only the system."""
+    import types
+
+    import torch
+
+    from sck_amd.sampler import Sampler
+    s = make()
+    rng = np.random.default_rng(4)
+    S = rng.standard_normal((2, s.n) + tuple(s.shape))
+    fake = types.SimpleNamespace(sys=s, unpack=lambda X: X)
+    fake._bond_gather = types.MethodType(Sampler._bond_gather, fake)
+    v = Sampler.local_field(fake, torch.as_tensor(S)).numpy()
+    np.testing.assert_allclose(v, _field_by_sites(s, S), rtol=0, atol=1e-12)
