@@ -18,59 +18,60 @@ import numpy as np
 from . import crystals as cr
 from .system import System
 
-__all__ = ["dmi", "surface", "surface_normals", "build_system", "nanodisk", "nanodisk_set", "MODELS"]
+__all__ = ["dmi", "surface", "surface_normals", "build_system", "nanodisk", "nanodisk_set", "MODELS", "STATE_KEYS"]
 
 MODELS = {"ising": 1, "xy": 2, "heisenberg": 3}
 
 
-def dmi(crystal: cr.Crystal, D: float) -> dict[int, np.ndarray]:
-    """Bulk-type DMI matrices A_lm = D sum_k r_k eps_klm on the in-plane first-shell bonds (r the unit bond)."""
-    eps = np.zeros((3, 3, 3))
-    for i, j, k in ((0, 1, 2), (1, 2, 0), (2, 0, 1)):
-        eps[i, j, k], eps[i, k, j] = 1.0, -1.0
-    out = {}
+def _first_shell(crystal: cr.Crystal):
+    """(bond index, bond, Cartesian bond vector padded to 3 components) of every first-shell bond."""
     for i, bd in enumerate(crystal.bonds):
-        if bd.shell != 1:
+        if bd.shell == 1:
+            offset = np.asarray(bd.delta, float) + crystal.basis[bd.alpha2] - crystal.basis[bd.alpha]
+            r = np.zeros(3)
+            r[:crystal.D] = offset @ crystal.vectors
+            yield i, bd, r
+
+
+def dmi(crystal: cr.Crystal, D: float) -> dict[int, np.ndarray]:
+    """Bulk-type (Bloch) DMI on the in-plane first-shell bonds. With u the unit bond vector the bond matrix is
+    A_lm = D sum_k u_k eps_klm, the skew matrix for which S_i . A S_j = D u . (S_i x S_j)."""
+    out = {}
+    for i, _, r in _first_shell(crystal):
+        if abs(r[2]) > 1e-9:                       # out-of-plane bonds carry none
             continue
-        r = np.zeros(3)
-        r[:crystal.D] = (np.asarray(bd.delta, float) + crystal.basis[bd.alpha2] - crystal.basis[bd.alpha]) @ crystal.vectors
-        if abs(r[2]) > 1e-9:
-            continue
-        r /= np.linalg.norm(r)
-        out[i] = D * np.einsum("k,klm->lm", r, eps)
+        x, y, z = r / np.linalg.norm(r)
+        out[i] = D * np.array([[0.0, z, -y], [-z, 0.0, x], [y, -x, 0.0]])
     return out
 
 
-def _shift(src: np.ndarray, delta, box, periodic, D):
-    """src (L3, L2, L1) moved so that element [z, y, x] holds src at (x, y, z) + delta, 0 outside an open box."""
-    out = src
-    valid = np.ones(src.shape, bool)
-    axes = {0: 2, 1: 1, 2: 0}
+def _partner(a: np.ndarray, delta, periodic, D: int) -> np.ndarray:
+    """Array p of a's shape (L3, L2, L1) with p[z, y, x] = a at (x, y, z) + delta: the index wraps across a
+    periodic face, and beyond an open face the value is 0."""
+    out = a
     for k in range(D):
-        d = delta[k]
+        d = int(delta[k])
         if d == 0:
             continue
-        out = np.roll(out, -d, axis=axes[k])
-        if not periodic[k]:
-            idx = np.arange(box[k]) + d
-            ok = (idx >= 0) & (idx < box[k])
-            shp = [1, 1, 1]
-            shp[axes[k]] = box[k]
-            valid = valid & ok.reshape(shp)
-    return np.where(valid, out, 0)
+        axis = 2 - k                                # lattice axis k is array axis 2 - k of (z, y, x)
+        L = a.shape[axis]
+        idx = np.arange(L) + d
+        if periodic[k]:
+            out = np.take(out, idx % L, axis=axis)
+        else:
+            inside = ((idx >= 0) & (idx < L)).reshape([L if j == axis else 1 for j in range(3)])
+            out = np.take(out, np.clip(idx, 0, L - 1), axis=axis) * inside
+    return out
 
 
 def surface(crystal: cr.Crystal, box, periodic, mask: np.ndarray) -> np.ndarray:
-    """Active sites with fewer active first-shell neighbours than a bulk site of their basis index."""
-    full = {a: 0 for a in range(crystal.b)}
-    count = np.zeros(mask.shape, int)
-    for bd in crystal.bonds:
-        if bd.shell != 1:
-            continue
-        full[bd.alpha] += 1
-        count[bd.alpha] += _shift(mask[bd.alpha2].astype(int), bd.delta, box, periodic, crystal.D)
-    bulk = np.stack([np.full(mask.shape[1:], full[a]) for a in range(crystal.b)])
-    return mask & (count < bulk)
+    """Active sites with fewer active first-shell neighbours than a bulk site of their basis index has bonds."""
+    bonds = [bd for _, bd, _ in _first_shell(crystal)]
+    bulk = np.bincount([bd.alpha for bd in bonds], minlength=crystal.b).reshape(-1, 1, 1, 1)
+    active = np.zeros(mask.shape, int)
+    for bd in bonds:
+        active[bd.alpha] += _partner(mask[bd.alpha2].astype(int), bd.delta, periodic, crystal.D)
+    return mask & (active < bulk)
 
 
 def surface_normals(crystal: cr.Crystal, box, periodic, mask: np.ndarray) -> np.ndarray:
@@ -79,12 +80,8 @@ def surface_normals(crystal: cr.Crystal, box, periodic, mask: np.ndarray) -> np.
     acc = np.zeros(mask.shape + (3,))
     first = np.zeros(mask.shape + (3,))
     has_first = np.zeros(mask.shape, bool)
-    for bd in crystal.bonds:
-        if bd.shell != 1:
-            continue
-        r = np.zeros(3)
-        r[:crystal.D] = (np.asarray(bd.delta, float) + crystal.basis[bd.alpha2] - crystal.basis[bd.alpha]) @ crystal.vectors
-        present = _shift(mask[bd.alpha2].astype(int), bd.delta, box, periodic, crystal.D).astype(bool)
+    for _, bd, r in _first_shell(crystal):
+        present = _partner(mask[bd.alpha2].astype(int), bd.delta, periodic, crystal.D).astype(bool)
         missing = mask[bd.alpha] & ~present
         acc[bd.alpha] += missing[..., None] * r
         new = missing & ~has_first[bd.alpha]
@@ -95,54 +92,72 @@ def surface_normals(crystal: cr.Crystal, box, periodic, mask: np.ndarray) -> np.
     return n * surface(crystal, box, periodic, mask)[..., None]
 
 
-def _f(theta, key, default=0.0) -> float:
-    v = theta.get(key, default)
-    return float(default if v is None else v)
+STATE_KEYS = ("name", "J", "D", "K1", "Ks", "K2", "H", "H_theta", "H_phi")
+
+
+def _number(theta: dict, key: str, default: float = 0.0) -> float:
+    """theta[key] as a float; an absent key or None reads as `default`."""
+    v = theta.get(key)
+    return float(default) if v is None else float(v)
+
+
+def _field(n: int, H: float, theta_deg: float, phi_deg: float):
+    """The field of strength H along polar angle theta and azimuth phi (degrees), or None for H = 0. Ising takes
+    H itself and XY the in-plane direction phi."""
+    if not H:
+        return None
+    t, p = np.deg2rad(theta_deg), np.deg2rad(phi_deg)
+    if n == 1:
+        comps = (H,)
+    elif n == 2:
+        comps = (H * np.cos(p), H * np.sin(p))
+    else:
+        comps = (H * np.sin(t) * np.cos(p), H * np.sin(t) * np.sin(p), H * np.cos(t))
+    return tuple(float(c) for c in comps)
+
+
+def _site_k1(c: cr.Crystal, cells, box, periodic, mask, K1: float, Ks: float) -> np.ndarray:
+    """Per-site K1 (float32): Ks on the surface sites of the active region, K1 on its other sites, 0 outside."""
+    active = np.ones(cells, bool) if mask is None else mask
+    return (np.where(surface(c, box, periodic, active), Ks, K1) * active).astype(np.float32)
 
 
 def build_system(model: str, crystal: str, box, periodic, theta: dict, mask=None, log=None) -> System:
-    """The System of one state theta; a periodic box without an admissible colouring falls back to open."""
-    model = str(model).strip().lower()
-    if model not in MODELS:
+    """The System of one state theta (the parameters of SCK/simulate.py:198-240: J shells, bulk DMI D, K1 with Ks
+    on surface sites, K2, field H at angles H_theta, H_phi). A periodic box with no admissible colouring is
+    built open instead, with a message to `log`."""
+    key = str(model).strip().lower()
+    if key not in MODELS:
         raise ValueError(f"unknown model {model!r}; known: {', '.join(MODELS)}")
-    n = MODELS[model]
-    J = {int(k): float(v) for k, v in (theta.get("J") or {1: 1.0}).items()}
-    D, K1, K2, H = _f(theta, "D"), _f(theta, "K1"), _f(theta, "K2"), _f(theta, "H")
-    Ks = _f(theta, "Ks", K1)
-    if n != 3 and any(_f(theta, k) for k in ("D", "K1", "Ks", "K2")):
-        raise ValueError(f"D, K1, Ks and K2 need the Heisenberg model; {model} carries J and H only")
-    unknown = set(theta) - {"name", "J", "D", "K1", "Ks", "K2", "H", "H_theta", "H_phi"}
+    n = MODELS[key]
+    J = {int(s): float(v) for s, v in (theta.get("J") or {1: 1.0}).items()}
+    D, K1, K2, H = (_number(theta, k) for k in ("D", "K1", "K2", "H"))
+    Ks = _number(theta, "Ks", K1)                  # the surface takes the bulk value unless Ks is given
+    if n != 3 and any(_number(theta, k) for k in ("D", "K1", "Ks", "K2")):
+        raise ValueError(f"D, K1, Ks and K2 need the Heisenberg model; {key} carries J and H only")
+    unknown = set(theta) - set(STATE_KEYS)
     if unknown:
-        raise ValueError(f"unknown parameters {sorted(unknown)}; known: J, D, K1, Ks, K2, H, H_theta, H_phi")
-    shells = tuple(sorted({s for s, v in J.items() if v != 0.0} | {1}))
-    c = cr.make(str(crystal), shells)
+        raise ValueError(f"unknown parameters {sorted(unknown)}; known: {', '.join(STATE_KEYS[1:])}")
+    c = cr.make(str(crystal), tuple(sorted({1} | {s for s, v in J.items() if v != 0.0})))
     box = tuple(int(v) for v in box)
-    periodic = tuple(bool(v) for v in periodic)
+    cells = (c.b, box[2] if c.D == 3 else 1, box[1], box[0])
     m = None if mask is None else np.asarray(mask, bool)
-    if m is not None and m.shape != (c.b, box[2] if c.D == 3 else 1, box[1], box[0]):
+    if m is not None and m.shape != cells:
         raise ValueError(f"mask shape {m.shape} does not match the box {box} of {c.name}")
-    t, p = np.deg2rad(_f(theta, "H_theta")), np.deg2rad(_f(theta, "H_phi"))
-    h = None
-    if H:
-        h = {1: (H,), 2: (H * np.cos(p), H * np.sin(p)),
-             3: (H * np.sin(t) * np.cos(p), H * np.sin(t) * np.sin(p), H * np.cos(t))}[n]
-        h = tuple(float(v) for v in h)
-    kw = dict(extra=dmi(c, D) if (n == 3 and D) else {}, h=h, cubic_k2=K2, mask=m)
-    name = theta.get("name") or f"{model} {c.name}"
-
-    def make(per):
-        k1 = None
-        if n == 3 and (K1 or Ks or K2):
-            mm = np.ones((c.b, box[2] if c.D == 3 else 1, box[1], box[0]), bool) if m is None else m
-            k1 = (np.where(surface(c, box, per, mm), Ks, K1) * mm).astype(np.float32)
-        return System(str(name), n, c, box, per, J, cubic_k1=k1, **kw)
-    try:
-        return make(periodic)
-    except ValueError as e:
-        if not any(periodic) or "colouring" not in str(e):
-            raise
-        (log or print)(f"  no admissible colouring for the periodic box {box}: using open boundaries")
-        return make((False,) * c.D)
+    terms = dict(extra=dmi(c, D) if (n == 3 and D) else {}, h=_field(n, H, _number(theta, "H_theta"),
+                 _number(theta, "H_phi")), cubic_k2=K2, mask=m)
+    name = str(theta.get("name") or f"{key} {c.name}")
+    anisotropic = n == 3 and bool(K1 or Ks or K2)
+    periodic = tuple(bool(v) for v in periodic)
+    tries = [periodic] + ([(False,) * c.D] if any(periodic) else [])
+    for i, per in enumerate(tries):
+        k1 = _site_k1(c, cells, box, per, m, K1, Ks) if anisotropic else None
+        try:
+            return System(name, n, c, box, per, J, cubic_k1=k1, **terms)
+        except ValueError as e:
+            if i == len(tries) - 1 or "colouring" not in str(e):
+                raise
+            (log or print)(f"  no admissible colouring for the periodic box {box}: using open boundaries")
 
 
 def nanodisk(radius: float, thickness: int, J1: float = 1.0, Ks: float = 0.3, D: float = 0.1,
