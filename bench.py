@@ -413,9 +413,10 @@ def learned_main(a) -> None:
                            "M": [float(r[k]["tau_M"][j]) for j in range(len(temps))]}
                        for k in ("uniform", "fixed", "learned")},
            "train_seconds": r["train_seconds"]}
-    f = REPO / "gpurun_out"
-    f.mkdir(exist_ok=True)
-    (f / f"{a.workload.replace('-', '_')}_comparison.json").write_text(json.dumps(r, default=float, indent=1))
+    if a.out_dir:
+        f = Path(a.out_dir)
+        f.mkdir(parents=True, exist_ok=True)
+        (f / f"{a.workload.replace('-', '_')}_comparison.json").write_text(json.dumps(r, default=float, indent=1))
     print(json.dumps(out, default=float), flush=True)
 
 
@@ -522,6 +523,9 @@ def main() -> None:
     ap.add_argument("--workload", default="xy", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="default workload: skip the learned-vs-fixed comparison")
+    ap.add_argument("--out-dir", default=None,
+                    help="learned comparisons: also write the full comparison (per-temperature tau_int, z-scores) "
+                         "as <workload>_comparison.json into this folder")
     ap.add_argument("--tile", default=None, help="replica-major kernels: class slots per tile, e.g. 8,8,4")
     ap.add_argument("--chunks", type=int, default=None, help="replica-major kernels: 64-replica chunks per wave")
     ap.add_argument("--resident-threads", type=int, default=None, help="LDS-resident kernel: threads per replica")
@@ -537,7 +541,7 @@ def main() -> None:
                     help="nanodisks: 'auto' (autotune) or kind/block/tile/order/grid, e.g. class/256/256/xcd/0")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build measures one GPU: the sweep path has no multi-GPU component "
+        print(json.dumps({"error": "bench.py measures one GPU: the sweep path has no multi-GPU component "
                           "(replica blocks shard across GPUs without communication); run with --gpus 1"}))
         sys.exit(2)
     import torch
