@@ -676,12 +676,12 @@ def ess_summary(log=None) -> dict:
     from sck_amd.ess import compare_proposals
     s, _, B, T, _ = _make("heisenberg")
     temps = np.unique(T.numpy())
-    r = compare_proposals(s, B, temps, warmup=200, equil=1000, therm=50, n_meas=400,
+    r = compare_proposals(s, B, temps, warmup=200, equil=1000, therm=50, n_meas=1000,
                           log=log or (lambda m: print(m, file=sys.stderr, flush=True)))
     return {"metric": "effective samples/s, learned vs fixed proposal",
             "workload": WORKLOADS["heisenberg"],
             "protocol": "train on 200 fixed-width warm-up sweeps; 1000 learned-width sweeps to one common "
-                        "equilibrium state; from it 50 + 400 sweeps of each proposal with an observation after "
+                        "equilibrium state; from it 50 + 1000 sweeps of each proposal with an observation after "
                         "every sweep; tau_int of E per temperature (Sokal window, c = 6)",
             "gain_E": r["gain_E"], "gain_E_vs_uniform": r["vs_uniform"]["gain_E"],
             "gain_E_vs_per_T_best_fixed": r["gain_E_vs_per_T_best"],
