@@ -672,3 +672,30 @@ def test_bitwise_needs_one_temperature_per_word():
     A.set_temperature(torch.tensor([2.0] * 32 + [3.0] * 32))
     with pytest.raises(ValueError, match="word"):
         A.set_temperature(torch.linspace(2.0, 3.0, 64))
+
+
+@pytest.mark.parametrize("name, want", [("ising", (-2.000000, -2.000000)), ("xy", (-1.812110, -2.105182)),
+                                        ("heisenberg", (-1.086448, -1.125840))])
+def test_reference_quickstart_on_the_gpu(name, want):
+    """The reference's quickstart end to end on the GPU Sampler (examples/quickstart.py:26-40, expected output
+    examples/quickstart.expected.txt:1-3): B = 2, seed 11, T = 1.5, initial spins from seed 2; the coloured
+    sweep equals the sequential chain after 3 sweeps, and the energy per site after 20 sweeps prints as the
+    reference's: exactly for Ising, and for XY and Heisenberg within 2e-6 (the reference allows their last
+    digits to differ on a GPU, whose float32 cos, sin and exp are not numpy's)."""
+    s = {"ising": lambda: sy.ising("square", 8), "xy": lambda: sy.xy("sc", 4),
+         "heisenberg": lambda: sy.heisenberg("triangular", 6)}[name]()
+    print(f"[gpu] quickstart {s.name}", flush=True)
+    A, R = Sampler(s, 2, DEV, seed=11), Sampler(s, 2, DEV, seed=11)
+    A.set_temperature(1.5)
+    R.set_temperature(1.5)
+    S1 = A.init_spins(seed=2)
+    S2 = S1.clone()
+    A.sweep(S1, 3)
+    R.reference_sweep(S2, 3)
+    assert torch.equal(S1, S2)
+    A.sweep(S1, 17)
+    e = (A.energy(S1) / s.n_active).cpu().numpy()
+    if s.n == 1:
+        assert (round(float(e[0]), 6), round(float(e[1]), 6)) == want
+    else:
+        assert np.allclose(e, want, rtol=0, atol=2e-6), e
