@@ -23,9 +23,11 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 from . import crystals as cr
 from . import simulate as SIM
-from .hamiltonian import MODELS, build_system
+from .hamiltonian import MODELS, build_system, surface
 
 __all__ = ["describe", "check", "run", "load", "Results"]
 
@@ -35,11 +37,24 @@ _RUN_KEYS = tuple(SIM.default_config())            # the settings a run folder r
 _EXTRA_KEYS = ("output_dir", "gpus")               # settings of `run` itself
 
 
+# the energy the state parameters set, term by term (the same model as the reference's, SCK/agent.py:27-30)
+_ENERGY = ("E = - sum over shells s of J_s sum over bonds <ij> of shell s of S_i . S_j"
+           " - sum over in-plane first-shell bonds of S_i . A_ij S_j, with A_ij the bulk DMI matrix D [u_ij]x of the"
+           " unit bond vector u_ij"
+           " + sum_i K1_i (sx^2 sy^2 + sy^2 sz^2 + sz^2 sx^2), with K1_i = Ks on surface sites and K1 elsewhere"
+           " + K2 sum_i (sx sy sz)^2 - sum_i H . S_i")
+_SHAPE_RULES = ("a condition on the coordinates of the chosen system, measured from the centre of the box, angles"
+                " in radians: numbers, the coordinate names, pi and the listed functions, arithmetic, comparisons,"
+                " and & | ^ ~ with parentheses around each comparison, e.g. (rho <= 10) & (abs(z) <= 2)")
+
+
 def describe() -> dict:
-    """The models, crystals, coordinate systems, state parameters, units and every default setting."""
-    return {"models": list(MODELS), "crystals": sorted(cr.KNOWN),
-            "coordinates": {k: list(v) for k, v in SIM.COORDS.items()}, "state_parameters": list(_STATE_KEYS),
-            "units": ["reduced", "meV"], "defaults": SIM.default_config()}
+    """The models, crystals, coordinate systems, shape language, state parameters, units, the energy they
+    define and every default setting (the keys of SCK/agent.py:32-46 that concern a run)."""
+    return {"hamiltonian": _ENERGY, "models": list(MODELS), "crystals": sorted(cr.KNOWN),
+            "coordinates": {k: list(v) for k, v in SIM.COORDS.items()},
+            "shape_functions": SIM.shape_functions(), "shape_rules": _SHAPE_RULES,
+            "state_parameters": list(_STATE_KEYS), "units": ["reduced", "meV"], "defaults": SIM.default_config()}
 
 
 def _settings(config, extra) -> dict:
@@ -100,19 +115,47 @@ def _rule_names(sim, issues):
 def _rule_states(sim, issues, report):
     found = issues.attempt(lambda: SIM.active_sites(sim["crystal"], sim["box"], sim["coords"], sim["shape"]))
     if found is None:
-        return
+        return None
     mask = found[1]
+    built = []
     for k, theta in enumerate(sim["states"]):
         s = issues.attempt(lambda: build_system(sim["model"], sim["crystal"], sim["box"], sim["periodic"], theta,
                                                 mask, log=lambda m: issues.warnings.append(m.strip())),
                            prefix=f"state {k}: ")
-        if s is not None and k == 0:
-            report.update(n_sites=int(mask.sum()), colouring=s.colouring.describe())
+        if s is not None:
+            built.append((theta["name"], s))
+            if k == 0:
+                report.update(n_sites=int(mask.sum()), colouring=s.colouring.describe())
+    return mask, built
+
+
+def _summary(sim, mask, built) -> dict:
+    """What a valid configuration amounts to (the report of SCK/agent.py:117-128): its sites (cells of the box,
+    active, on the surface, in the bulk), each state's colouring and the periodicity it runs with, the
+    temperature ladder, and the size of the run."""
+    c = cr.make(str(sim["crystal"]))
+    box = [int(v) for v in sim["box"]]
+    cells = c.b * int(np.prod(box))
+    active = int(mask.sum())
+    on_surface = int(surface(c, box, [bool(v) for v in sim["periodic"]], mask).sum())
+    temps = SIM.temperature_ladder(sim["T_max"], sim["T_min"], sim["T_step"])
+    chains = len(sim["states"]) * int(sim["replicas"])
+    saved = len(range(0, len(temps), max(1, int(sim["config_every"]))))
+    sweeps = int(sim["thermalization"]) + int(sim["measurement"])
+    return {"sites": {"box": cells, "active": active, "surface": on_surface, "bulk": active - on_surface},
+            "states": [{"name": name, "colours": s.colouring.n_colours, "colouring": s.colouring.describe(),
+                        "periodic_used": list(s.periodic)} for name, s in built],
+            "temperatures": {"count": len(temps), "first": float(temps[0]), "last": float(temps[-1])},
+            "budget": {"chains": chains, "spin_updates": chains * len(temps) * active * sweeps,
+                       "configurations_saved_per_chain": saved,
+                       "output_gb": round(SIM.size_estimate_gb(len(sim["states"]), int(sim["replicas"]), saved,
+                                                              active), 4)}}
 
 
 def check(config: dict | None = None, **settings) -> dict:
-    """Validate a configuration without running it: {"ok", "errors", "warnings", "config"}. A bad configuration
-    comes back as errors; it never raises."""
+    """Validate a configuration without running it: {"ok", "errors", "warnings", "config"}, and for a valid one
+    also "sites", "states", "temperatures" and "budget". A bad configuration comes back as errors; it never
+    raises."""
     cfg = _settings(config, settings)
     issues = _Issues()
     unknown = sorted(set(cfg) - set(_RUN_KEYS) - set(_EXTRA_KEYS))
@@ -124,9 +167,12 @@ def check(config: dict | None = None, **settings) -> dict:
     report = {}
     for rule in (_rule_ladder, _rule_counts, _rule_names):
         rule(sim, issues)
-    _rule_states(sim, issues, report)
-    return dict(ok=not issues.errors, errors=issues.errors, warnings=issues.warnings,
-                config={**cfg, **report, **sim})
+    found = _rule_states(sim, issues, report)
+    out = dict(ok=not issues.errors, errors=issues.errors, warnings=issues.warnings,
+               config={**cfg, **report, **sim})
+    if out["ok"]:
+        out.update(_summary(sim, *found))
+    return out
 
 
 def _visible_gpus(wanted) -> list[int]:
@@ -238,6 +284,31 @@ class Results:
                 ("Cv", self.specific_heat), ("chi", self.susceptibility))
         return [{"state": name, "T": float(T), **{k: float(v[i, j]) for k, v in cols}}
                 for i, name in enumerate(self.names) for j, T in enumerate(self.temperatures)]
+
+    def plot(self, path=None, show: bool = True):
+        """Energy, |M|, Cv and chi per site against temperature, one line per state, in a 2 x 2 figure; saved
+        to `path` when one is given (needs matplotlib)."""
+        import matplotlib.pyplot as plt
+        panels = (("energy per site", self.energy_per_site), ("|M| per site", self.magnetization_per_site),
+                  ("Cv per site", self.specific_heat), ("chi per site", self.susceptibility))
+        fig, axes = plt.subplots(2, 2, figsize=(11, 7), sharex=True)
+        for ax, (label, values) in zip(axes.flat, panels):
+            for name, row in zip(self.names, values):
+                ax.plot(self.temperatures, row, marker="o", ms=3, lw=1.2, label=name)
+            ax.set_ylabel(label)
+            ax.grid(alpha=0.25)
+        for ax in axes[1]:
+            ax.set_xlabel(f"T ({self.temperature_unit})")
+        axes[0, 0].legend(fontsize=9)
+        c = self.config
+        fig.suptitle(f"{c.get('model')} on {c.get('crystal')} {list(c.get('box', []))}: {len(self.names)} state(s), "
+                     f"{c.get('replicas')} replicas")
+        fig.tight_layout()
+        if path is not None:
+            fig.savefig(path, dpi=150)
+        if show:
+            plt.show()
+        return fig
 
     def __repr__(self) -> str:
         T, u = self.temperatures, self.temperature_unit

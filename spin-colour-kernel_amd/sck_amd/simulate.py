@@ -60,6 +60,11 @@ _OPS = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, as
         ast.GtE: operator.ge, ast.Eq: operator.eq, ast.NotEq: operator.ne}
 
 
+def shape_functions() -> list[str]:
+    """The names a shape condition may call, and the constant pi."""
+    return list(_FUNCS) + ["pi"]
+
+
 class _Shape(ast.NodeVisitor):
     """A shape condition, checked and evaluated in one walk of its syntax tree: numbers, the coordinate names,
     the functions of _FUNCS (and pi), arithmetic, comparisons and & | ^ ~ are allowed, anything else is refused

@@ -111,6 +111,36 @@ def test_check_reports_errors_without_raising():
 
 
 # ------------------------------------------------------------------------------------------------ GPU
+def test_check_reports_the_size_of_a_valid_run():
+    """For a valid configuration `check` also reports its sites, each state's colouring, the ladder and the
+    budget (the keys of SCK/agent.py:117-128, read by the reference's nanodisk notebook): here a Heisenberg
+    disc of radius 3.5 in an open 9 x 9 x 3 sc box, the surface counted site by site (fewer than six active
+    neighbours), the budget from the settings."""
+    cfg = dict(model="heisenberg", crystal="sc", box=[9, 9, 3], periodic=[False, False, False],
+               coords="cylindrical", shape="rho <= 3.5", states=[dict(name="disc", J={1: 1.0}, K1=0.1)],
+               T_max=2.0, T_min=0.5, T_step=0.5, thermalization=30, measurement=20, measure_every=5,
+               replicas=8, config_every=2)
+    r = api.check(cfg)
+    assert r["ok"], r["errors"]
+    xs = np.arange(9) - 4.0
+    disc = {(x, y) for x in range(9) for y in range(9) if math.hypot(xs[x], xs[y]) <= 3.5}
+    active = {(x, y, z) for (x, y) in disc for z in range(3)}
+    nbrs = ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))
+    on_surface = sum(1 for (x, y, z) in active if sum((x + a, y + b, z + c) in active for a, b, c in nbrs) < 6)
+    assert r["sites"] == {"box": 243, "active": len(active), "surface": on_surface,
+                          "bulk": len(active) - on_surface}
+    assert [st["name"] for st in r["states"]] == ["disc"] and r["states"][0]["colours"] == 2
+    assert r["states"][0]["periodic_used"] == [False, False, False]
+    assert r["temperatures"] == {"count": 4, "first": 2.0, "last": 0.5}
+    assert r["budget"] == {"chains": 8, "spin_updates": 8 * 4 * len(active) * 50,
+                           "configurations_saved_per_chain": 2,
+                           "output_gb": round(12.0 * 8 * 2 * len(active) / 1e9, 4)}
+    assert not any(k in api.check(dict(cfg, replicas=0)) for k in ("sites", "budget"))
+    d = api.describe()
+    assert {"hamiltonian", "shape_functions", "shape_rules", "state_parameters", "defaults"} <= set(d)
+    assert "rho" in d["coordinates"]["cylindrical"] and "sqrt" in d["shape_functions"]
+
+
 @pytest.mark.gpu
 def test_run_end_to_end(tmp_path):
     res = api.run(_cfg(), output_dir=tmp_path, verbose=False)
