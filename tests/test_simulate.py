@@ -138,6 +138,8 @@ def test_check_reports_the_size_of_a_valid_run():
     assert not any(k in api.check(dict(cfg, replicas=0)) for k in ("sites", "budget"))
     d = api.describe()
     assert {"hamiltonian", "shape_functions", "shape_rules", "state_parameters", "defaults"} <= set(d)
+    import sck_amd as sck                           # the reference's short interface: import sck; sck.check(...)
+    assert sck.check is api.check and sck.run is api.run and sck.Results is api.Results
     assert "rho" in d["coordinates"]["cylindrical"] and "sqrt" in d["shape_functions"]
 
 
