@@ -285,9 +285,9 @@ DEV void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32
 }
 
 // philox10 for the counter (c0, c1, 0, 0) and key (k0, 0), the form every sweep draw uses (c1 = 0 unless the
-// counter is 64-bit). Round 1 has one product (the other multiplies c2 = 0) and leaves c0 = c1 ^ k0, c1 = 0, so
-// round 2's second product depends on c1 and the key alone and is uniform across a wave when they are; rounds
-// 3-10 are the general ones.
+// counter is 64-bit). The first Philox round has one product (the other multiplies c2 = 0) and leaves
+// c0 = c1 ^ k0, c1 = 0, so the second round's second product depends on c1 and the key alone and is uniform
+// across a wave when they are; the third to tenth rounds are the general ones.
 DEV void philox10_z(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0) {
     const uint64_t q = (uint64_t)0xD2511F53u * (uint64_t)c0;
     const uint64_t p0 = (uint64_t)0xCD9E8D57u * (uint64_t)(uint32_t)(q >> 32);
