@@ -529,7 +529,7 @@ def main() -> None:
     ap.add_argument("--tile", default=None, help="replica-major kernels: class slots per tile, e.g. 8,8,4")
     ap.add_argument("--chunks", type=int, default=None, help="replica-major kernels: 64-replica chunks per wave")
     ap.add_argument("--resident-threads", type=int, default=None, help="LDS-resident kernel: threads per replica")
-    ap.add_argument("--tile-dims", default=None, help="tiled sweep kernel: cells per tile along x and y, e.g. 64,64")
+    ap.add_argument("--tile-dims", default=None, help="tiled sweep kernel: cells per tile along x and y, and optionally threads per workgroup, e.g. 64,64 or 128,32,512")
     ap.add_argument("--batch", type=int, default=None, help="learned-xy: replicas (default 512; 16 temperatures)")
     ap.add_argument("--layout", default="auto", choices=("auto", "reference", "replica"),
                     help="spin storage (replica: lanes over replicas, per-class launches)")

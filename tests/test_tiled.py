@@ -60,6 +60,11 @@ SYSTEMS = {
     # four colours, float4 rows with a cell's two components side by side in LDS, partial tiles
     "xy_triangular_100": (lambda: sy.xy("triangular", 100), {}, None),
     "xy_triangular_open_masked_cone": (lambda: _xy_triangular_open(100, 64), {"proposal": "cone"}, None),
+    # 512-thread workgroups: taller tiles (up to 80 KB of LDS), partial tiles along both axes
+    "xy_triangular_300_512_threads": (lambda: sy.xy("triangular", 300), {}, (128, 64, 512)),
+    "heis_square_100x90_512_threads": (
+        lambda: sy.System("Heis square", 3, cr.make("square"), (100, 90), (True, True), {1: 1.0},
+                          h=(0.0, 0.0, 0.3), uniaxial=(0.2, (0.0, 0.0, 1.0))), {}, (64, 48, 512)),
 }
 
 
@@ -69,6 +74,18 @@ def test_tiled_plan_geometry():
     assert len(g.classes) == 4                     # 2048 mod 3 != 0: the seam condition forces four colours
     assert g.tiled_plan() == (128, 32, 4, 4, 1, 1)
     assert G._Gen(s, G.MODE_F32, 2048, "uniform", tile_dims=(64, 64)).tiled_plan() == (64, 64, 4, 4, 1, 1)
+    # 512 threads: a 128 x 64 tile and its halo (136 x 72 cells, 78 KB) fit the larger LDS budget
+    g512 = G._Gen(s, G.MODE_F32, 2048, "uniform", tile_dims=(128, 64, 512))
+    assert g512.tiled_plan() == (128, 64, 4, 4, 1, 1) and g512.TILE_THREADS == 512
+    assert not G.fits_tiled(s, G.MODE_F32, (128, 64))                    # 78 KB exceeds the default 64 KB
+    m = G.generate(s, G.MODE_F32, 2048, tiled=True)
+    assert m.tile_dims == (128, 32, 512) and "amdgpu_waves_per_eu(1, 6)" in m.src
+    assert G.generate(s, G.MODE_F32, 2048, tiled=True, tile_dims=(128, 32, 256)).tiled.block[0] == 256
+    m512 = G.generate(s, G.MODE_F32, 2048, tiled=True, tile_dims=(128, 64, 512))
+    assert m512.tile_dims == (128, 64, 512) and m512.tiled.block[0] == 512
+    assert "amdgpu_waves_per_eu(1, 4)" in m512.src                        # two workgroups of 8 waves per CU
+    with pytest.raises(ValueError, match="256, 512 or 1024"):
+        G._Gen(s, G.MODE_F32, 2048, "uniform", tile_dims=(128, 64, 384))
     # Heisenberg triangular, three colours: the tile is rounded to the colouring period and to float4 columns
     assert G._Gen(sy.heisenberg("triangular", 300), G.MODE_F32, 8, "uniform").tiled_plan()[:2] == (120, 30)
     assert not G.fits_tiled(sy.xy("triangular", 12), G.MODE_F32)          # smaller than a tile and its halo
